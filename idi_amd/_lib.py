@@ -1,0 +1,246 @@
+"""ctypes binding of libidisim.so (the C ABI in include/idisim.h).
+
+Device memory and streams come from PyTorch (``torch.cuda``): tensors are
+passed to the library as raw device pointers, and work is enqueued on the
+current torch stream.  There is no CPU fallback: every entry point raises
+``RuntimeError`` when the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libidisim.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "idisim.h")
+
+# constants of include/idisim.h
+NF, SCALE, SECONDORDER, ROTATE = 1, 2, 4, 8
+PREC_FP64, PREC_FP32SPLIT = 0, 1
+OUT_FIELD, OUT_INTENSITY = 0, 1
+OPT_AUTOTUNE, OPT_TILE, OPT_SPLIT, OPT_SUMMODE, OPT_GRAPH, OPT_TOL_PPB, OPT_FALLBACK = 1, 2, 3, 4, 5, 6, 7
+E_PRECISION = -4
+
+
+class GeometryT(ctypes.Structure):
+    _fields_ = [
+        ("n0", ctypes.c_int64),
+        ("n1", ctypes.c_int64),
+        ("pixelsize", ctypes.c_double),
+        ("detz", ctypes.c_double),
+        ("k", ctypes.c_double),
+        ("detoffset", ctypes.c_double * 2),
+        ("rot", ctypes.c_double * 9),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class PlanT(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+        ("tile_rows", ctypes.c_int32),
+        ("tile_cols", ctypes.c_int32),
+        ("split", ctypes.c_int32),
+        ("summode", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+        ("fallback", ctypes.c_int32),
+        ("phase_bound", ctypes.c_double),
+        ("tuned_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def declared_functions(header=HEADER):
+    """Names of the functions include/idisim.h declares."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(idisim_\w+)\s*\(", text, re.M)))
+
+
+def load():
+    """Load libidisim.so (after torch, so that both use one HIP runtime)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        import torch  # noqa: F401  (torch's libamdhip64 must be the process's HIP runtime)
+    except ImportError:
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m idi_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i64, dbl, ip = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    lib.idisim_version.restype = ip
+    lib.idisim_last_error.restype = ctypes.c_char_p
+    lib.idisim_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(GeometryT), ip, ip]
+    lib.idisim_destroy.argtypes = [vp]
+    lib.idisim_set_option.argtypes = [vp, ip, i64]
+    lib.idisim_get_plan.argtypes = [vp, ctypes.POINTER(PlanT)]
+    lib.idisim_forward.argtypes = [vp, vp, vp, i64, dbl, vp, ip, vp]
+    lib.idisim_forward_batched.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_int64), i64, dbl, vp, ip, vp]
+    lib.idisim_adjoint.argtypes = [vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp, vp]
+    lib.idisim_extent.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), vp]
+    for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
+                 "idisim_forward_batched", "idisim_adjoint", "idisim_extent"):
+        getattr(lib, name).restype = ip
+    if lib.idisim_version() != 1:
+        raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
+    _lib = lib
+    return lib
+
+
+class IdisimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"idisim error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != 0:
+        raise IdisimError(rc, _lib.idisim_last_error().decode(errors="replace"))
+
+
+def _require_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("idi_amd needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path")
+    return torch
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Handle:
+    """One geometry + precision on one device; owns the library handle."""
+
+    def __init__(self, n0, n1, pixelsize, detz, k, detoffset=(0.0, 0.0), rot=None, nf=False, scale=False,
+                 secondorder=False, precision=PREC_FP32SPLIT, device=None):
+        torch = _require_cuda()
+        lib = load()
+        self.device = torch.cuda.current_device() if device is None else int(device)
+        g = GeometryT()
+        g.n0, g.n1 = int(n0), int(n1)
+        g.pixelsize, g.detz, g.k = float(pixelsize), float(detz), float(k)
+        g.detoffset[0], g.detoffset[1] = float(detoffset[0]), float(detoffset[1])
+        flags = (NF if nf else 0) | (SCALE if scale else 0) | (SECONDORDER if secondorder else 0)
+        M = np.eye(3) if rot is None else np.asarray(rot, dtype=np.float64).reshape(3, 3)
+        if rot is not None:
+            flags |= ROTATE
+        for i, v in enumerate(M.ravel()):
+            g.rot[i] = float(v)
+        g.flags = flags
+        self.geom = g
+        self.shape = (int(n0), int(n1))
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        check(lib.idisim_create(ctypes.byref(self._h), ctypes.byref(g), int(precision), self.device))
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.idisim_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def field_dtype(self):
+        import torch
+
+        return torch.complex128 if self.precision == PREC_FP64 else torch.complex64
+
+    @property
+    def real_dtype(self):
+        import torch
+
+        return torch.float64 if self.precision == PREC_FP64 else torch.float32
+
+    def set_option(self, option, value):
+        check(self._lib.idisim_set_option(self._h, int(option), int(value)))
+
+    def plan(self):
+        p = PlanT()
+        check(self._lib.idisim_get_plan(self._h, ctypes.byref(p)))
+        return p.as_dict()
+
+    def forward(self, emit, amp=None, out=None, kind=OUT_FIELD, extent=0.0):
+        """emit: (N, 4) float64 CUDA tensor; amp: (N,) float64 or None; returns `out`."""
+        import torch
+
+        if emit.dtype != torch.float64 or emit.dim() != 2 or emit.shape[1] != 4 or not emit.is_contiguous():
+            raise ValueError("emitters must be a contiguous (N, 4) float64 tensor")
+        if amp is not None and (amp.dtype != torch.float64 or amp.shape != (emit.shape[0],) or not amp.is_contiguous()):
+            raise ValueError("amplitudes must be a contiguous (N,) float64 tensor")
+        if out is None:
+            out = torch.empty(self.shape, dtype=self.field_dtype if kind == OUT_FIELD else self.real_dtype,
+                              device=emit.device)
+        self._check_out(out, kind, 1)
+        check(self._lib.idisim_forward(self._h, _ptr(emit), _ptr(amp), int(emit.shape[0]), float(extent), _ptr(out),
+                                       int(kind), _stream()))
+        return out
+
+    def forward_batched(self, emit, offsets, amp=None, out=None, kind=OUT_FIELD, extent=0.0):
+        """emit: concatenated (sum N_s, 4) float64; offsets: host int64 (nshots + 1)."""
+        import torch
+
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        nshots = len(offs) - 1
+        if nshots < 1 or offs[0] != 0 or offs[-1] != emit.shape[0]:
+            raise ValueError("offsets must start at 0 and end at the number of emitters")
+        if emit.dtype != torch.float64 or emit.dim() != 2 or emit.shape[1] != 4 or not emit.is_contiguous():
+            raise ValueError("emitters must be a contiguous (N, 4) float64 tensor")
+        if amp is not None and (amp.dtype != torch.float64 or amp.shape != (emit.shape[0],) or not amp.is_contiguous()):
+            raise ValueError("amplitudes must be a contiguous (N,) float64 tensor")
+        if out is None:
+            out = torch.empty((nshots,) + self.shape,
+                              dtype=self.field_dtype if kind == OUT_FIELD else self.real_dtype, device=emit.device)
+        self._check_out(out, kind, nshots)
+        self._offs = offs  # keep alive for the call
+        check(self._lib.idisim_forward_batched(
+            self._h, _ptr(emit), _ptr(amp), offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nshots,
+            float(extent), _ptr(out), int(kind), _stream()))
+        return out
+
+    def adjoint(self, emit, field, dLdI, amp=None, extent=0.0, want_amp=True):
+        """Gradients (pos (N,3), phase (N,), amp (N,) or None) in float32 for a loss L(|E|^2)."""
+        import torch
+
+        n = emit.shape[0]
+        if field.dtype != torch.complex64 or tuple(field.shape) != self.shape or not field.is_contiguous():
+            raise ValueError("field must be the contiguous complex64 (n0, n1) output of forward")
+        if dLdI.dtype != torch.float32 or tuple(dLdI.shape) != self.shape or not dLdI.is_contiguous():
+            raise ValueError("dLdI must be a contiguous float32 (n0, n1) tensor")
+        gpos = torch.empty((n, 3), dtype=torch.float32, device=emit.device)
+        gph = torch.empty((n,), dtype=torch.float32, device=emit.device)
+        gamp = torch.empty((n,), dtype=torch.float32, device=emit.device) if want_amp else None
+        check(self._lib.idisim_adjoint(self._h, _ptr(emit), _ptr(amp), int(n), float(extent), _ptr(field),
+                                       _ptr(dLdI), _ptr(gpos), _ptr(gph), _ptr(gamp), _stream()))
+        return gpos, gph, gamp
+
+    def _check_out(self, out, kind, nshots):
+        import torch
+
+        want = self.field_dtype if kind == OUT_FIELD else self.real_dtype
+        shape = self.shape if nshots == 1 and out.dim() == 2 else (nshots,) + self.shape
+        if out.dtype != want or tuple(out.shape) != shape or not out.is_contiguous() or not out.is_cuda:
+            raise ValueError(f"output must be a contiguous CUDA {want} tensor of shape {shape}")

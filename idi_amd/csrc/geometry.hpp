@@ -1,0 +1,155 @@
+// FP64 detector geometry and the range-reduced phase decomposition used by the
+// FP32-split kernels.  Shared by device code (kernels.hip) and host code (the
+// planner in idisim.hip), so everything here is __host__ __device__.
+//
+// Phase models (reference): idi/simulation/cpu.py:42-66, sim.cu:43-95.
+//
+// Decomposition used by the FP32-split path (revolutions, 1 rev = 2 pi rad,
+// krev = k / 2 pi).  With R_p = |r_p|, rh_p = r_p / R_p, u_p = rh_p - z_hat,
+// s_pj = rh_p . r_j, h_pj^2 = |rh_p x r_j|^2 and D_pj = |r_p - r_j|:
+//   near field:  theta/2pi = krev (R_p - z_p)                    per pixel
+//                          - krev u_p . r_j + phi_j / 2pi         linear part
+//                          + krev h^2 / (D + R_p - s)             curvature w_pj
+//   (exact:  D - R + s = h^2 / (D + R - s), so D - z_p + z_j = (R_p - z_p) - u_p.r_j + w_pj)
+//   far field:   theta/2pi = -krev u_p . r_j + phi_j / 2pi [+ krev (x^2+y^2) / (2 R_p)]
+// A workgroup owns a tile of pixels with centre c.  Writing u_p = u_c + du_p:
+//   tau_pj = Phi_p + A_cj + lam_p W_cj + dq_p . r'_cj
+//   A_cj  = frac(-krev u_c . r_j + phi_j / 2pi)          FP64, reduced mod 1 rev
+//   W_cj  = krev w_cj (nf) or krev (x^2+y^2)/(2 R_c) (secondorder)
+//   lam_p = R_c / R_p,   dq_p = -krev du_p
+//   r'_cj = r_j (1 + sigma_cj),  sigma_cj = (s_cj - w_cj) / D_cj  (nf), 0 (ff)
+// The nf lines are the first-order expansion of w in (s, R) about the tile
+// centre (dw/ds = (w - s)/D, dw/dR = -w/D); the host planner bounds the
+// remainder (plan_bound in idisim.hip).  Far field (with or without
+// secondorder) is exact up to rounding.  Only A, W, r', dq, lam are rounded to
+// FP32; |dq . r'| is a few revolutions, so the FP32 phase keeps ~1e-7 rev.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace idisim {
+
+constexpr double kInv2Pi = 0.15915494309189533576888376337251436;
+constexpr double kTwoPi = 6.28318530717958647692528676655900577;
+
+struct Geom {
+    double pix, detz, k, krev;
+    double off0, off1, half0, half1;   // detoffset, n0/2 and n1/2 (true division)
+    double M[9];
+    int n0, n1;
+    int rot, nf, scale, secondorder;
+};
+
+struct Frame {
+    double R;       // |r_p|
+    double rh[3];   // r_p / R
+    double u[3];    // rh - z_hat, u_z formed without cancellation
+    double RmZ;     // R - z_p, formed without cancellation
+};
+
+// cpu.py:76-80 at a (possibly fractional) pixel index
+__host__ __device__ inline void pixel_pos(const Geom& g, double fx, double fy, double r[3])
+{
+    const double dx = (fx - g.half0) * g.pix + g.off0;
+    const double dy = (fy - g.half1) * g.pix + g.off1;
+    const double dz = g.detz;
+    if (g.rot) {
+        r[0] = g.M[0] * dx + g.M[1] * dy + g.M[2] * dz;
+        r[1] = g.M[3] * dx + g.M[4] * dy + g.M[5] * dz;
+        r[2] = g.M[6] * dx + g.M[7] * dy + g.M[8] * dz;
+    } else {
+        r[0] = dx;
+        r[1] = dy;
+        r[2] = dz;
+    }
+}
+
+__host__ __device__ inline Frame make_frame(const double r[3])
+{
+    Frame f;
+    const double rxy2 = r[0] * r[0] + r[1] * r[1];
+    f.R = sqrt(rxy2 + r[2] * r[2]);
+    const double iR = 1.0 / f.R;
+    f.rh[0] = r[0] * iR;
+    f.rh[1] = r[1] * iR;
+    f.rh[2] = r[2] * iR;
+    f.u[0] = f.rh[0];
+    f.u[1] = f.rh[1];
+    if (r[2] > 0.0) {
+        f.RmZ = rxy2 / (f.R + r[2]);
+        f.u[2] = -f.RmZ * iR;
+    } else {
+        f.RmZ = f.R - r[2];
+        f.u[2] = f.rh[2] - 1.0;
+    }
+    return f;
+}
+
+__host__ __device__ inline Frame pixel_frame(const Geom& g, double fx, double fy)
+{
+    double r[3];
+    pixel_pos(g, fx, fy, r);
+    return make_frame(r);
+}
+
+// Per (tile centre, emitter) record of the FP32-split kernels, formed in FP64.
+struct SplitRec {
+    float x, y, z, A;   // r'_cj and the reduced linear phase
+    float W, amp;       // curvature / second-order coefficient, amplitude
+    float sigma, s;     // kept for the adjoint: sigma_cj and s_cj (nf)
+};
+
+__host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c, double ex,
+                                                   double ey, double ez, double ephi, double ea)
+{
+    const double ucr = c.u[0] * ex + c.u[1] * ey + c.u[2] * ez;
+    const double tau = -g.krev * ucr + ephi * kInv2Pi;
+    double W = 0.0, sig = 0.0, am = ea, s = 0.0;
+    if (g.nf) {
+        s = ucr + ez;
+        const double hx = c.rh[1] * ez - c.rh[2] * ey;
+        const double hy = c.rh[2] * ex - c.rh[0] * ez;
+        const double hz = c.rh[0] * ey - c.rh[1] * ex;
+        const double h2 = hx * hx + hy * hy + hz * hz;
+        const double a = c.R - s;
+        const double D = sqrt(a * a + h2);
+        const double w = (a >= 0.0) ? h2 / (D + a) : D - a;
+        W = g.krev * w;
+        sig = (s - w) / D;
+        if (g.scale) am = ea * (c.R / D);
+    } else if (g.secondorder) {
+        W = g.krev * (ex * ex + ey * ey) / (2.0 * c.R);
+    }
+    const double A = tau - rint(tau);
+    const double f1 = 1.0 + sig;
+    SplitRec r;
+    r.x = (float)(ex * f1);
+    r.y = (float)(ey * f1);
+    r.z = (float)(ez * f1);
+    r.A = (float)A;
+    r.W = (float)W;
+    r.amp = (float)am;
+    r.sigma = (float)sig;
+    r.s = (float)s;
+    return r;
+}
+
+// Work item of the persistent batched kernels: one (shot, pixel tile).
+struct WorkItem {
+    int32_t shot;
+    int32_t tile;
+    int64_t e0, e1;   // emitter range [e0, e1) in the concatenated array
+};
+
+// Pixel-tile variants: lanes form a 16 x 16 grid, each lane owns PX x PY pixels.
+constexpr int kNumTiles = 4;
+__host__ __device__ constexpr int tile_px(int v) { return v >= 2 ? 2 : 1; }
+__host__ __device__ constexpr int tile_py(int v) { return v == 0 ? 1 : (v == 3 ? 4 : 2); }
+__host__ __device__ constexpr int tile_rows(int v) { return 16 * tile_px(v); }
+__host__ __device__ constexpr int tile_cols(int v) { return 16 * tile_py(v); }
+
+constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
+constexpr int kChunk = 256;   // emitters staged in LDS per round
+
+}  // namespace idisim

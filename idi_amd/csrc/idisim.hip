@@ -1,0 +1,853 @@
+// C ABI of the MI355X direct-sum wavefield library (declared in include/idisim.h).
+//
+// Host-side responsibilities:
+//   - geometry set-up in FP64 (replaces the closure of cpu.py:6-40 and the
+//     per-call flag/compile step of cuda.py:53-86);
+//   - the FP32-split error planner: for every tile variant, an a-priori bound
+//     of the per-term phase error (FP32 rounding + nf linearisation + sin/cos)
+//     from the detector geometry and the emitter extent; tiles whose bound
+//     exceeds the tolerance are not used, and when none qualifies the handle
+//     runs the FP64 kernel (or returns IDISIM_E_PRECISION);
+//   - the autotuner: times candidate (tile, emitter split / blocks per CU)
+//     plans with hipEvents on the caller's stream and caches the fastest per
+//     (output kind, amplitudes, batched, log2 n);
+//   - scratch, work-item tables, and hipGraph capture/replay of the
+//     persistent batched launch.
+#include "../../include/idisim.h"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace idisim;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define IDISIM_HIP(expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(IDISIM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Largest absolute error of v_sin_f32 / v_cos_f32 on [-0.5, 0.5] rev, measured
+// on MI355X against FP64 sin/cos (in revolutions of equivalent phase).
+constexpr double kSinCosErrRev = 2.0e-7;
+
+struct TuneKey {
+    int kind, has_amp, batched, nbucket;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(kind, has_amp, batched, nbucket) < std::tie(o.kind, o.has_amp, o.batched, o.nbucket);
+    }
+};
+
+struct Choice {
+    int tile = 2;
+    int split = 1;       // emitter splits (single shot) or blocks per CU (batched)
+    double ms = 0.0;
+};
+
+}  // namespace
+
+struct idisim_handle {
+    int device = 0;
+    int precision = IDISIM_PREC_FP32SPLIT;
+    int num_cus = 256;
+    idisim_geometry geom{};
+    Geom g{};
+    // options
+    int autotune = 1, force_tile = -1, force_split = 0, summode = 0, use_graph = 1, fallback = 1;
+    double tol_rev = 3e-6;
+    // planner tables
+    double du_max[kNumTiles] = {0, 0, 0, 0};
+    double dR_max[kNumTiles] = {0, 0, 0, 0};
+    double Rmin = 0.0, Rmax = 0.0;
+    // device scratch
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    unsigned long long* d_ext = nullptr;
+    // batched work items
+    WorkItem* d_items = nullptr;
+    size_t items_cap = 0;
+    WorkItem* h_items = nullptr;  // pinned staging
+    std::vector<int64_t> items_offsets;
+    int items_tile = -1;
+    int nitems = 0;
+    int* d_counter = nullptr;
+    // graph of the batched launch
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gkey;
+    std::map<TuneKey, Choice> tuned;
+    idisim_plan plan{};
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+void planner_tables(idisim_handle* h)
+{
+    const Geom& g = h->g;
+    h->Rmin = std::numeric_limits<double>::infinity();
+    h->Rmax = 0.0;
+    for (int v = 0; v < kNumTiles; ++v) {
+        const int TX = tile_rows(v), TY = tile_cols(v);
+        double du = 0.0, dR = 0.0;
+        for (int tx0 = 0; tx0 < g.n0; tx0 += TX) {
+            for (int ty0 = 0; ty0 < g.n1; ty0 += TY) {
+                const int xe = std::min(tx0 + TX, g.n0) - 1, ye = std::min(ty0 + TY, g.n1) - 1;
+                const Frame c = pixel_frame(g, 0.5 * (tx0 + xe), 0.5 * (ty0 + ye));
+                const double xs[3] = {(double)tx0, (double)((tx0 + xe) / 2), (double)xe};
+                const double ys[3] = {(double)ty0, (double)((ty0 + ye) / 2), (double)ye};
+                h->Rmin = std::min(h->Rmin, c.R);
+                h->Rmax = std::max(h->Rmax, c.R);
+                for (double fx : xs) {
+                    for (double fy : ys) {
+                        const Frame f = pixel_frame(g, fx, fy);
+                        const double d0 = f.u[0] - c.u[0], d1 = f.u[1] - c.u[1], d2 = f.u[2] - c.u[2];
+                        du = std::max(du, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+                        dR = std::max(dR, std::fabs(f.R - c.R));
+                        h->Rmin = std::min(h->Rmin, f.R);
+                        h->Rmax = std::max(h->Rmax, f.R);
+                    }
+                }
+            }
+        }
+        // corners and edge midpoints sample the tile; u is smooth, margin 5%
+        h->du_max[v] = 1.05 * du;
+        h->dR_max[v] = 1.05 * dR;
+    }
+}
+
+// A-priori bound of the per-term phase error (revolutions) of tile variant v
+// for emitters within |r_j| <= rho.
+double plan_bound(const idisim_handle* h, int v, double rho)
+{
+    const Geom& g = h->g;
+    const double du = h->du_max[v], dR = h->dR_max[v];
+    const double Rlo = h->Rmin - dR - rho;
+    if (!(Rlo > 0.0)) return std::numeric_limits<double>::infinity();
+    const double Rhi = h->Rmax + dR + rho;
+    const double ds = du * rho;
+    double wmax = 0.0, sig = 0.0;
+    if (g.nf) {
+        wmax = rho * rho / (2.0 * Rlo);
+        sig = (rho + wmax) / Rlo;
+    } else if (g.secondorder) {
+        wmax = rho * rho / (2.0 * Rlo);
+    }
+    const double W = g.krev * wmax;
+    const double B = g.krev * ds * (1.0 + sig);
+    const double tau = 0.5 + W + B;
+    const double ulp = std::ldexp(1.0, std::ilogb(tau) - 23);
+    const double u24 = std::ldexp(1.0, -24);
+    double e = 2.0 * ulp;                // four FMA roundings of tau, each <= ulp/2
+    e += 6.0 * u24 * B;                  // dq and r' rounded to FP32
+    e += 0.5 * u24 + 2.0 * u24 * W;      // A, W and lam rounded to FP32
+    if (g.nf) {
+        const double R3 = Rlo * Rlo * Rlo;
+        const double wss = Rhi * Rhi / R3;
+        const double wsR = (Rhi * rho + rho * rho) / R3;
+        const double wRR = 2.0 * wmax * Rhi / R3;
+        const double lin = 0.5 * wss * ds * ds + wsR * ds * dR + 0.5 * wRR * dR * dR +
+                           wmax * dR * (dR + rho) / (Rlo * Rlo);
+        e += g.krev * lin;
+        if (g.scale) e += (ds / Rlo + rho * dR / (Rlo * Rlo)) * kInv2Pi;  // amplitude, as phase
+    }
+    e += kSinCosErrRev;
+    return e;
+}
+
+size_t out_elem_bytes(int precision, int kind)
+{
+    if (precision == IDISIM_PREC_FP64) return kind == IDISIM_OUT_INTENSITY ? sizeof(double) : 2 * sizeof(double);
+    return kind == IDISIM_OUT_INTENSITY ? sizeof(float) : 2 * sizeof(float);
+}
+
+int ensure_scratch(idisim_handle* h, size_t bytes)
+{
+    if (bytes <= h->scratch_bytes) return IDISIM_OK;
+    if (h->scratch) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(h->scratch);
+        h->scratch = nullptr;
+        h->scratch_bytes = 0;
+    }
+    if (hipMalloc(&h->scratch, bytes) != hipSuccess) {
+        h->scratch = nullptr;
+        return fail(IDISIM_E_NOMEM, "scratch allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    h->scratch_bytes = bytes;
+    return IDISIM_OK;
+}
+
+int measure_extent(idisim_handle* h, const double* emit, int64_t n, hipStream_t st, double* rho)
+{
+    if (n <= 0) {
+        *rho = 0.0;
+        return IDISIM_OK;
+    }
+    if (!h->d_ext) IDISIM_HIP(hipMalloc(&h->d_ext, sizeof(unsigned long long)));
+    IDISIM_HIP(launch_extent(emit, n, h->d_ext, st));
+    unsigned long long bits = 0;
+    IDISIM_HIP(hipMemcpyAsync(&bits, h->d_ext, sizeof(bits), hipMemcpyDeviceToHost, st));
+    IDISIM_HIP(hipStreamSynchronize(st));
+    double r2;
+    std::memcpy(&r2, &bits, sizeof(r2));
+    *rho = std::sqrt(r2);
+    return IDISIM_OK;
+}
+
+int nbucket(int64_t n)
+{
+    int b = 0;
+    while (n > 1) {
+        n >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+// Tile variants of the FP32-split kernels whose error bound meets the tolerance.
+std::vector<int> eligible_tiles(const idisim_handle* h, double rho, double* best_bound)
+{
+    std::vector<int> out;
+    *best_bound = std::numeric_limits<double>::infinity();
+    for (int v = 0; v < kNumTiles; ++v) {
+        const double b = plan_bound(h, v, rho);
+        *best_bound = std::min(*best_bound, b);
+        if (b <= h->tol_rev) out.push_back(v);
+    }
+    return out;
+}
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer()
+    {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+    }
+    ~Timer()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// ------------------------------ single shot --------------------------------
+struct FwdCall {
+    const double* emit;
+    const double* amp;
+    int64_t n;
+    void* out;
+    int kind;
+    hipStream_t st;
+    bool split_path;  // FP32-split kernels (false: FP64 kernels)
+    bool out_f32;     // FP64 kernels writing a complex64 / float32 output (fallback)
+};
+
+int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
+{
+    const Geom& g = h->g;
+    const size_t npix = (size_t)g.n0 * g.n1;
+    if (c.split_path) {
+        const bool w = g.nf || g.secondorder;
+        const bool a = (c.amp != nullptr) || (g.nf && g.scale);
+        float4* partial = nullptr;
+        if (splits > 1) {
+            int rc = ensure_scratch(h, (size_t)splits * npix * sizeof(float4));
+            if (rc) return rc;
+            partial = static_cast<float4*>(h->scratch);
+        }
+        IDISIM_HIP(launch_fwd_split(tile, w, a, h->summode, g, c.emit, c.amp, c.n, splits, partial, c.out, c.kind,
+                                    c.st));
+        if (splits > 1) IDISIM_HIP(launch_finalize_split(g, partial, splits, c.out, c.kind, c.st));
+        return IDISIM_OK;
+    }
+    // FP64 kernels; a complex64/float32 output (fallback of an FP32-split
+    // handle) always goes through the finalize kernel, which converts.
+    double4* partial = nullptr;
+    if (splits > 1 || c.out_f32) {
+        int rc = ensure_scratch(h, (size_t)splits * npix * sizeof(double4));
+        if (rc) return rc;
+        partial = static_cast<double4*>(h->scratch);
+    }
+    IDISIM_HIP(launch_fwd_fp64(tile, g, c.emit, c.amp, c.n, splits, partial, c.out, c.kind, c.st));
+    if (partial != nullptr) {
+        IDISIM_HIP(launch_finalize_fp64(g, partial, splits, c.out, c.kind | (c.out_f32 ? 2 : 0), c.st));
+    }
+    return IDISIM_OK;
+}
+
+std::vector<int> split_candidates(const idisim_handle* h, int64_t n, int tile)
+{
+    const int64_t maxs = std::max<int64_t>(1, (n + kChunk - 1) / kChunk);
+    const int tiles = ((h->g.n0 + tile_rows(tile) - 1) / tile_rows(tile)) *
+                      ((h->g.n1 + tile_cols(tile) - 1) / tile_cols(tile));
+    std::vector<int> out;
+    for (int s = 1; s <= 64 && s <= maxs; s *= 2) {
+        out.push_back(s);
+        if ((int64_t)tiles * s >= 16LL * h->num_cus) break;  // 16 workgroups per CU is plenty
+    }
+    return out;
+}
+
+int heuristic_split(const idisim_handle* h, int64_t n, int tile)
+{
+    const std::vector<int> c = split_candidates(h, n, tile);
+    const int tiles = ((h->g.n0 + tile_rows(tile) - 1) / tile_rows(tile)) *
+                      ((h->g.n1 + tile_cols(tile) - 1) / tile_cols(tile));
+    for (int s : c)
+        if ((int64_t)tiles * s >= 4LL * h->num_cus) return s;
+    return c.back();
+}
+
+int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
+{
+    const TuneKey key{c.kind, c.amp != nullptr, 0, nbucket(c.n)};
+    Choice ch;
+    auto it = h->tuned.find(key);
+    const bool forced = h->force_tile >= 0 || h->force_split > 0;
+    if (forced) {
+        ch.tile = h->force_tile >= 0 ? h->force_tile : tiles.front();
+        ch.split = h->force_split > 0 ? h->force_split : heuristic_split(h, c.n, ch.tile);
+    } else if (it != h->tuned.end()) {
+        ch = it->second;
+    } else if (h->autotune) {
+        // time candidates on (at most) ~2e10 pairs of this call's work
+        const size_t npix = (size_t)h->g.n0 * h->g.n1;
+        int64_t nt = c.n;
+        const double cap = c.split_path ? 2e10 : 2e9;
+        if ((double)nt * (double)npix > cap) nt = std::max<int64_t>(kChunk, (int64_t)(cap / (double)npix));
+        FwdCall ct = c;
+        ct.n = nt;
+        Timer tm;
+        double best = std::numeric_limits<double>::infinity();
+        for (int v : tiles) {
+            for (int s : split_candidates(h, nt, v)) {
+                int rc = run_forward(h, ct, v, s);  // warm-up
+                if (rc) return rc;
+                float ms = 0.f;
+                IDISIM_HIP(hipEventRecord(tm.a, c.st));
+                rc = run_forward(h, ct, v, s);
+                if (rc) return rc;
+                IDISIM_HIP(hipEventRecord(tm.b, c.st));
+                IDISIM_HIP(hipEventSynchronize(tm.b));
+                IDISIM_HIP(hipEventElapsedTime(&ms, tm.a, tm.b));
+                if (ms < best) {
+                    best = ms;
+                    ch.tile = v;
+                    ch.split = s;
+                    ch.ms = ms;
+                }
+            }
+        }
+        // the split that won on nt emitters, rescaled to n by workgroup count
+        if (nt < c.n) {
+            const std::vector<int> full = split_candidates(h, c.n, ch.tile);
+            ch.split = std::min(ch.split, full.back());
+        }
+        h->tuned[key] = ch;
+    } else {
+        // heuristic: 2x2 pixels per lane if allowed, enough splits for 4 WG/CU
+        ch.tile = tiles.front();
+        for (int v : {2, 1, 3, 0})
+            if (std::find(tiles.begin(), tiles.end(), v) != tiles.end()) {
+                ch.tile = v;
+                break;
+            }
+        ch.split = heuristic_split(h, c.n, ch.tile);
+    }
+    const int64_t maxs = std::max<int64_t>(1, (c.n + kChunk - 1) / kChunk);
+    if (ch.split > maxs) ch.split = (int)maxs;
+    int rc = run_forward(h, c, ch.tile, ch.split);
+    if (rc) return rc;
+    h->plan.tile = ch.tile;
+    h->plan.tile_rows = tile_rows(ch.tile);
+    h->plan.tile_cols = tile_cols(ch.tile);
+    h->plan.split = ch.split;
+    h->plan.tuned_ms = ch.ms;
+    return IDISIM_OK;
+}
+
+// ------------------------------ batched ------------------------------------
+int build_items(idisim_handle* h, const int64_t* offsets, int64_t nshots, int tile, hipStream_t st)
+{
+    const Geom& g = h->g;
+    std::vector<int64_t> key(offsets, offsets + nshots + 1);
+    if (tile == h->items_tile && key == h->items_offsets && h->d_items) return IDISIM_OK;
+    const int tx = (g.n0 + tile_rows(tile) - 1) / tile_rows(tile);
+    const int ty = (g.n1 + tile_cols(tile) - 1) / tile_cols(tile);
+    const int64_t ntiles = (int64_t)tx * ty;
+    const int64_t nitems = ntiles * nshots;
+    if (nitems > std::numeric_limits<int>::max()) return fail(IDISIM_E_INVALID, "too many work items");
+    std::vector<WorkItem> items;
+    items.reserve((size_t)nitems);
+    for (int64_t s = 0; s < nshots; ++s)
+        for (int64_t t = 0; t < ntiles; ++t) items.push_back(WorkItem{(int32_t)s, (int32_t)t, offsets[s], offsets[s + 1]});
+    // longest first: the queue tail is made of the shortest items
+    std::stable_sort(items.begin(), items.end(),
+                     [](const WorkItem& a, const WorkItem& b) { return (a.e1 - a.e0) > (b.e1 - b.e0); });
+    IDISIM_HIP(hipStreamSynchronize(st));  // the staging buffer may still feed a previous copy
+    if ((size_t)nitems > h->items_cap) {
+        if (h->d_items) (void)hipFree(h->d_items);
+        if (h->h_items) (void)hipHostFree(h->h_items);
+        h->d_items = nullptr;
+        h->h_items = nullptr;
+        h->items_cap = 0;
+        if (hipMalloc(&h->d_items, (size_t)nitems * sizeof(WorkItem)) != hipSuccess)
+            return fail(IDISIM_E_NOMEM, "work item table allocation failed");
+        if (hipHostMalloc(&h->h_items, (size_t)nitems * sizeof(WorkItem), hipHostMallocDefault) != hipSuccess)
+            return fail(IDISIM_E_NOMEM, "pinned work item staging allocation failed");
+        h->items_cap = (size_t)nitems;
+    }
+    std::memcpy(h->h_items, items.data(), items.size() * sizeof(WorkItem));
+    IDISIM_HIP(hipMemcpyAsync(h->d_items, h->h_items, items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+    if (!h->d_counter) IDISIM_HIP(hipMalloc(&h->d_counter, sizeof(int)));
+    h->items_offsets = std::move(key);
+    h->items_tile = tile;
+    h->nitems = (int)nitems;
+    return IDISIM_OK;
+}
+
+struct BatchCall {
+    const double* emit;
+    const double* amp;
+    void* out;
+    int kind;
+    hipStream_t st;
+    bool split_path;
+};
+
+int launch_batched(idisim_handle* h, const BatchCall& c, int tile, int blocks_per_cu, hipStream_t st)
+{
+    const Geom& g = h->g;
+    const int nblocks = std::max(1, blocks_per_cu * h->num_cus);
+    if (c.split_path) {
+        const bool w = g.nf || g.secondorder;
+        const bool a = (c.amp != nullptr) || (g.nf && g.scale);
+        IDISIM_HIP(launch_batched_split(tile, w, a, h->summode, g, c.emit, c.amp, h->d_items, h->nitems, h->d_counter,
+                                        nblocks, c.out, c.kind, st));
+    } else {
+        IDISIM_HIP(launch_batched_fp64(tile, g, c.emit, c.amp, h->d_items, h->nitems, h->d_counter, nblocks, c.out,
+                                       c.kind, st));
+    }
+    return IDISIM_OK;
+}
+
+int occupancy(idisim_handle* h, const BatchCall& c, int tile)
+{
+    const Geom& g = h->g;
+    if (c.split_path)
+        return occupancy_batched_split(tile, g.nf || g.secondorder, (c.amp != nullptr) || (g.nf && g.scale),
+                                       h->summode);
+    return occupancy_batched_fp64(tile);
+}
+
+int run_batched(idisim_handle* h, const BatchCall& c, int tile, int bpc)
+{
+    if (!h->use_graph) {
+        h->plan.graph = 0;
+        return launch_batched(h, c, tile, bpc, c.st);
+    }
+    std::vector<uintptr_t> key = {(uintptr_t)c.emit, (uintptr_t)c.amp, (uintptr_t)c.out, (uintptr_t)c.kind,
+                                  (uintptr_t)h->d_items, (uintptr_t)h->nitems, (uintptr_t)tile, (uintptr_t)bpc,
+                                  (uintptr_t)h->summode, (uintptr_t)c.split_path};
+    if (key != h->gkey || !h->gexec) {
+        if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+        if (h->graph) (void)hipGraphDestroy(h->graph);
+        h->gexec = nullptr;
+        h->graph = nullptr;
+        h->gkey.clear();
+        if (!h->cap_stream) IDISIM_HIP(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+        IDISIM_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_batched(h, c, tile, bpc, h->cap_stream);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(h->cap_stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(IDISIM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        h->graph = graph;
+        IDISIM_HIP(hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0));
+        h->gkey = key;
+    }
+    IDISIM_HIP(hipGraphLaunch(h->gexec, c.st));
+    h->plan.graph = 1;
+    return IDISIM_OK;
+}
+
+int fill_zero(void* out, size_t bytes, hipStream_t st)
+{
+    if (bytes) IDISIM_HIP(hipMemsetAsync(out, 0, bytes, st));
+    return IDISIM_OK;
+}
+
+}  // namespace
+
+// ============================== C ABI =======================================
+extern "C" {
+
+int idisim_version(void) { return IDISIM_VERSION; }
+
+const char* idisim_last_error(void) { return g_err.c_str(); }
+
+int idisim_create(idisim_handle** out, const idisim_geometry* geom, int precision, int device)
+{
+    g_err.clear();
+    if (!out || !geom) return fail(IDISIM_E_INVALID, "idisim_create: NULL argument");
+    *out = nullptr;
+    if (geom->n0 <= 0 || geom->n1 <= 0 || geom->n0 > (1 << 20) || geom->n1 > (1 << 20) ||
+        geom->n0 * geom->n1 > (int64_t)1 << 31)
+        return fail(IDISIM_E_INVALID, "idisim_create: detector size out of range");
+    if (!(geom->k > 0.0) || !std::isfinite(geom->k) || !std::isfinite(geom->pixelsize) ||
+        !std::isfinite(geom->detz))
+        return fail(IDISIM_E_INVALID, "idisim_create: k, pixelsize and detz must be finite, k > 0");
+    if (precision != IDISIM_PREC_FP64 && precision != IDISIM_PREC_FP32SPLIT)
+        return fail(IDISIM_E_INVALID, "idisim_create: unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(IDISIM_E_HIP, "idisim_create: no HIP device available");
+    if (device < 0 || device >= ndev) return fail(IDISIM_E_INVALID, "idisim_create: invalid device");
+    DeviceGuard guard(device);
+    idisim_handle* h = new idisim_handle();
+    h->device = device;
+    h->precision = precision;
+    h->geom = *geom;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        h->num_cus = prop.multiProcessorCount;
+    Geom& g = h->g;
+    g.pix = geom->pixelsize;
+    g.detz = geom->detz;
+    g.k = geom->k;
+    g.krev = geom->k * kInv2Pi;
+    g.off0 = geom->detoffset[0];
+    g.off1 = geom->detoffset[1];
+    g.half0 = (double)geom->n0 / 2.0;
+    g.half1 = (double)geom->n1 / 2.0;
+    for (int i = 0; i < 9; ++i) g.M[i] = geom->rot[i];
+    g.n0 = (int)geom->n0;
+    g.n1 = (int)geom->n1;
+    g.rot = (geom->flags & IDISIM_ROTATE) ? 1 : 0;
+    g.nf = (geom->flags & IDISIM_NF) ? 1 : 0;
+    g.scale = (geom->flags & IDISIM_SCALE) ? 1 : 0;
+    g.secondorder = (geom->flags & IDISIM_SECONDORDER) ? 1 : 0;
+    planner_tables(h);
+    h->plan.precision = precision;
+    *out = h;
+    return IDISIM_OK;
+}
+
+int idisim_destroy(idisim_handle* h)
+{
+    if (!h) return IDISIM_OK;
+    DeviceGuard guard(h->device);
+    (void)hipDeviceSynchronize();
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+    if (h->scratch) (void)hipFree(h->scratch);
+    if (h->d_ext) (void)hipFree(h->d_ext);
+    if (h->d_items) (void)hipFree(h->d_items);
+    if (h->h_items) (void)hipHostFree(h->h_items);
+    if (h->d_counter) (void)hipFree(h->d_counter);
+    delete h;
+    return IDISIM_OK;
+}
+
+int idisim_set_option(idisim_handle* h, int option, int64_t value)
+{
+    if (!h) return fail(IDISIM_E_INVALID, "idisim_set_option: NULL handle");
+    switch (option) {
+        case IDISIM_OPT_AUTOTUNE: h->autotune = value ? 1 : 0; break;
+        case IDISIM_OPT_TILE:
+            if (value < -1 || value >= kNumTiles) return fail(IDISIM_E_INVALID, "tile variant out of range");
+            h->force_tile = (int)value;
+            break;
+        case IDISIM_OPT_SPLIT:
+            if (value < 0 || value > 4096) return fail(IDISIM_E_INVALID, "split out of range");
+            h->force_split = (int)value;
+            break;
+        case IDISIM_OPT_SUMMODE:
+            if (value != 0 && value != 1) return fail(IDISIM_E_INVALID, "summode must be 0 or 1");
+            h->summode = (int)value;
+            break;
+        case IDISIM_OPT_GRAPH: h->use_graph = value ? 1 : 0; break;
+        case IDISIM_OPT_TOL_PPB:
+            if (value <= 0) return fail(IDISIM_E_INVALID, "tolerance must be positive");
+            h->tol_rev = (double)value * 1e-9;
+            break;
+        case IDISIM_OPT_FALLBACK: h->fallback = value ? 1 : 0; break;
+        default: return fail(IDISIM_E_INVALID, "unknown option " + std::to_string(option));
+    }
+    h->tuned.clear();
+    return IDISIM_OK;
+}
+
+int idisim_get_plan(const idisim_handle* h, idisim_plan* out)
+{
+    if (!h || !out) return fail(IDISIM_E_INVALID, "idisim_get_plan: NULL argument");
+    *out = h->plan;
+    return IDISIM_OK;
+}
+
+int idisim_extent(const double* emitters, int64_t n, double* extent_out, void* stream)
+{
+    if (!extent_out || (n > 0 && !emitters) || n < 0) return fail(IDISIM_E_INVALID, "idisim_extent: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (n == 0) {
+        *extent_out = 0.0;
+        return IDISIM_OK;
+    }
+    unsigned long long* d = nullptr;
+    IDISIM_HIP(hipMalloc(&d, sizeof(unsigned long long)));
+    hipError_t e = launch_extent(emitters, n, d, st);
+    unsigned long long bits = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(IDISIM_E_HIP, std::string("idisim_extent: ") + hipGetErrorString(e));
+    double r2;
+    std::memcpy(&r2, &bits, sizeof(r2));
+    *extent_out = std::sqrt(r2);
+    return IDISIM_OK;
+}
+
+// Decide FP32-split vs FP64 for a call; sets the plan's precision fields.
+static int choose_path(idisim_handle* h, double rho, std::vector<int>* tiles, bool* split_path)
+{
+    h->plan.fallback = 0;
+    h->plan.summode = h->summode;
+    h->plan.graph = 0;
+    if (h->precision == IDISIM_PREC_FP64) {
+        *split_path = false;
+        *tiles = {0, 1, 2, 3};
+        h->plan.precision = IDISIM_PREC_FP64;
+        h->plan.phase_bound = 0.0;
+        return IDISIM_OK;
+    }
+    double best = 0.0;
+    std::vector<int> ok = eligible_tiles(h, rho, &best);
+    if (h->force_tile >= 0) {
+        h->plan.phase_bound = plan_bound(h, h->force_tile, rho);
+        *tiles = {h->force_tile};
+        *split_path = true;
+        h->plan.precision = IDISIM_PREC_FP32SPLIT;
+        return IDISIM_OK;
+    }
+    if (ok.empty()) {
+        if (!h->fallback)
+            return fail(IDISIM_E_PRECISION, "FP32-split phase error bound " + std::to_string(best) +
+                                                " rev exceeds the tolerance " + std::to_string(h->tol_rev) +
+                                                " rev for this geometry and emitter extent");
+        *split_path = false;
+        *tiles = {0, 1, 2, 3};
+        h->plan.fallback = 1;
+        h->plan.precision = IDISIM_PREC_FP64;
+        h->plan.phase_bound = 0.0;
+        return IDISIM_OK;
+    }
+    double b = std::numeric_limits<double>::infinity();
+    for (int v : ok) b = std::min(b, plan_bound(h, v, rho));
+    h->plan.phase_bound = b;
+    *tiles = ok;
+    *split_path = true;
+    h->plan.precision = IDISIM_PREC_FP32SPLIT;
+    return IDISIM_OK;
+}
+
+int idisim_forward(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
+                   void* out, int out_kind, void* stream)
+{
+    g_err.clear();
+    if (!h) return fail(IDISIM_E_INVALID, "idisim_forward: NULL handle");
+    if (!out) return fail(IDISIM_E_INVALID, "idisim_forward: NULL output");
+    if (n < 0 || (n > 0 && !emitters)) return fail(IDISIM_E_INVALID, "idisim_forward: bad emitter arguments");
+    if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
+        return fail(IDISIM_E_INVALID, "idisim_forward: unknown output kind");
+    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward: extent must be finite");
+    DeviceGuard guard(h->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t npix = (size_t)h->g.n0 * h->g.n1;
+    if (n == 0) return fill_zero(out, npix * out_elem_bytes(h->precision, out_kind), st);
+    double rho = extent;
+    if (!(rho > 0.0)) {
+        int rc = measure_extent(h, emitters, n, st, &rho);
+        if (rc) return rc;
+    }
+    std::vector<int> tiles;
+    bool split_path = true;
+    int rc = choose_path(h, rho, &tiles, &split_path);
+    if (rc) return rc;
+    FwdCall c{emitters, amplitudes, n, out, out_kind, st, split_path,
+              h->precision == IDISIM_PREC_FP32SPLIT && !split_path};
+    return plan_and_run(h, c, tiles);
+}
+
+int idisim_forward_batched(idisim_handle* h, const double* emitters, const double* amplitudes,
+                           const int64_t* offsets, int64_t nshots, double extent, void* out, int out_kind,
+                           void* stream)
+{
+    g_err.clear();
+    if (!h) return fail(IDISIM_E_INVALID, "idisim_forward_batched: NULL handle");
+    if (!out || !offsets || nshots <= 0) return fail(IDISIM_E_INVALID, "idisim_forward_batched: bad arguments");
+    if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
+        return fail(IDISIM_E_INVALID, "idisim_forward_batched: unknown output kind");
+    if (offsets[0] != 0) return fail(IDISIM_E_INVALID, "idisim_forward_batched: offsets[0] must be 0");
+    for (int64_t s = 0; s < nshots; ++s)
+        if (offsets[s + 1] < offsets[s])
+            return fail(IDISIM_E_INVALID, "idisim_forward_batched: offsets must be non-decreasing");
+    const int64_t total = offsets[nshots];
+    if (total > 0 && !emitters) return fail(IDISIM_E_INVALID, "idisim_forward_batched: NULL emitters");
+    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward_batched: extent must be finite");
+    DeviceGuard guard(h->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double rho = extent;
+    if (!(rho > 0.0)) {
+        int rc = measure_extent(h, emitters, total, st, &rho);
+        if (rc) return rc;
+    }
+    std::vector<int> tiles;
+    bool split_path = true;
+    int rc = choose_path(h, rho, &tiles, &split_path);
+    if (rc) return rc;
+    if (h->precision == IDISIM_PREC_FP32SPLIT && !split_path)
+        return fail(IDISIM_E_PRECISION,
+                    "idisim_forward_batched: FP32-split error bound not met; use an FP64 handle for this geometry");
+    BatchCall c{emitters, amplitudes, out, out_kind, st, split_path};
+
+    const TuneKey key{out_kind, amplitudes != nullptr, 1, nbucket(std::max<int64_t>(1, total / nshots))};
+    Choice ch;
+    auto it = h->tuned.find(key);
+    if (h->force_tile >= 0 || h->force_split > 0) {
+        ch.tile = h->force_tile >= 0 ? h->force_tile : tiles.front();
+        ch.split = h->force_split > 0 ? h->force_split : occupancy(h, c, ch.tile);
+    } else if (it != h->tuned.end()) {
+        ch = it->second;
+    } else if (h->autotune) {
+        Timer tm;
+        double best = std::numeric_limits<double>::infinity();
+        for (int v : tiles) {
+            rc = build_items(h, offsets, nshots, v, st);
+            if (rc) return rc;
+            const int occ = occupancy(h, c, v);
+            std::vector<int> bpcs = {occ};
+            if (occ > 2) bpcs.push_back(occ / 2);
+            for (int bpc : bpcs) {
+                rc = launch_batched(h, c, v, bpc, st);  // warm-up
+                if (rc) return rc;
+                float ms = 0.f;
+                IDISIM_HIP(hipEventRecord(tm.a, st));
+                rc = launch_batched(h, c, v, bpc, st);
+                if (rc) return rc;
+                IDISIM_HIP(hipEventRecord(tm.b, st));
+                IDISIM_HIP(hipEventSynchronize(tm.b));
+                IDISIM_HIP(hipEventElapsedTime(&ms, tm.a, tm.b));
+                if (ms < best) {
+                    best = ms;
+                    ch.tile = v;
+                    ch.split = bpc;
+                    ch.ms = ms;
+                }
+            }
+        }
+        h->tuned[key] = ch;
+    } else {
+        ch.tile = tiles.front();
+        for (int v : {2, 1, 3, 0})
+            if (std::find(tiles.begin(), tiles.end(), v) != tiles.end()) {
+                ch.tile = v;
+                break;
+            }
+        ch.split = occupancy(h, c, ch.tile);
+    }
+    rc = build_items(h, offsets, nshots, ch.tile, st);
+    if (rc) return rc;
+    rc = run_batched(h, c, ch.tile, ch.split);
+    if (rc) return rc;
+    h->plan.tile = ch.tile;
+    h->plan.tile_rows = tile_rows(ch.tile);
+    h->plan.tile_cols = tile_cols(ch.tile);
+    h->plan.split = ch.split;
+    h->plan.tuned_ms = ch.ms;
+    return IDISIM_OK;
+}
+
+int idisim_adjoint(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
+                   const void* field, const float* dLdI, float* grad_pos, float* grad_phase, float* grad_amp,
+                   void* stream)
+{
+    g_err.clear();
+    if (!h) return fail(IDISIM_E_INVALID, "idisim_adjoint: NULL handle");
+    if (n < 0 || (n > 0 && (!emitters || !grad_pos || !grad_phase)) || !field || !dLdI)
+        return fail(IDISIM_E_INVALID, "idisim_adjoint: bad arguments");
+    if (h->precision != IDISIM_PREC_FP32SPLIT)
+        return fail(IDISIM_E_UNSUPPORTED, "idisim_adjoint: only FP32-split handles have an adjoint");
+    if (h->g.scale) return fail(IDISIM_E_UNSUPPORTED, "idisim_adjoint: the 1/r 'scale' model has no adjoint");
+    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_adjoint: extent must be finite");
+    if (n == 0) return IDISIM_OK;
+    DeviceGuard guard(h->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double rho = extent;
+    if (!(rho > 0.0)) {
+        int rc = measure_extent(h, emitters, n, st, &rho);
+        if (rc) return rc;
+    }
+    const double bound = plan_bound(h, 0, rho);  // the adjoint streams 16 x 16 pixel tiles
+    if (bound > h->tol_rev)
+        return fail(IDISIM_E_PRECISION, "idisim_adjoint: FP32-split error bound " + std::to_string(bound) +
+                                            " rev exceeds the tolerance for this geometry");
+    const int ntiles = ((h->g.n0 + 15) / 16) * ((h->g.n1 + 15) / 16);
+    const int64_t egroups = (n + 511) / 512;
+    int64_t ps = (8LL * h->num_cus + egroups - 1) / egroups;
+    if (ps > ntiles) ps = ntiles;
+    if (ps < 1) ps = 1;
+    const int per = (int)((ntiles + ps - 1) / ps);
+    const int splits = (ntiles + per - 1) / per;
+    int rc = ensure_scratch(h, (size_t)splits * (size_t)n * 5 * sizeof(double));
+    if (rc) return rc;
+    IDISIM_HIP(launch_adjoint(h->g.nf || h->g.secondorder, h->g, emitters, amplitudes, n,
+                              static_cast<const float2*>(field), dLdI, (int)ps, static_cast<double*>(h->scratch),
+                              grad_pos, grad_phase, grad_amp, st));
+    h->plan.precision = IDISIM_PREC_FP32SPLIT;
+    h->plan.tile = 0;
+    h->plan.tile_rows = 16;
+    h->plan.tile_cols = 16;
+    h->plan.split = splits;
+    h->plan.phase_bound = bound;
+    h->plan.fallback = 0;
+    h->plan.graph = 0;
+    return IDISIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,673 @@
+// Forward kernels of the direct-sum wavefield for gfx950 (wave64, 256 CUs).
+//
+// FP32-split kernels (reference counterpart: wfkernelf, sim.cu:101-144)
+//   - a workgroup (256 lanes = 4 waves) owns a tile of pixels; each lane owns
+//     PX x PY neighbouring pixels;
+//   - emitters stream through LDS in chunks of 256: each lane forms ONE
+//     emitter's record in FP64 (range-reduced linear phase A, curvature W,
+//     scaled position r', amplitude), all lanes then read every record as an
+//     LDS broadcast;
+//   - per pair: tau = A + lam W + dq . r' (4 FMA), v_sin_f32 / v_cos_f32 on
+//     revolutions, 2 FMA into per-chunk FP32 partial sums, which are merged
+//     into the running sum with Kahan compensation after every chunk (or, with
+//     SUM == 1, Kahan on every term as cpu.py:97-105 does);
+//   - emitters can be split over workgroups (blockIdx.y); partial sums then go
+//     to a scratch buffer and a finalize kernel combines them in FP64.
+// FP64 kernel (reference counterpart: wfkerneld, sim.cu:146-149, and the numba
+// kernel cpu.py:68-107): the literal formula in FP64 with Kahan per term, same
+// tiling and LDS staging; it is the "plain FP64" comparator and the fallback.
+#include "kernels.hpp"
+
+namespace idisim {
+
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float v)
+{
+    const float y = v - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+__device__ __forceinline__ void kahan_add_d(double& sum, double& comp, double v)
+{
+    const double y = v - comp;
+    const double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// Pixel-index centre of the part of a tile that lies on the detector.
+__device__ __forceinline__ void tile_centre(const Geom& g, int tx0, int ty0, int TX, int TY,
+                                            double& cx, double& cy)
+{
+    const int xe = min(tx0 + TX, g.n0) - 1;
+    const int ye = min(ty0 + TY, g.n1) - 1;
+    cx = 0.5 * (double)(tx0 + xe);
+    cy = 0.5 * (double)(ty0 + ye);
+}
+
+// Final per-pixel step of the FP32-split path: the per-pixel phase
+// krev (R_p - z_p) of the near-field model (dropped by |E|^2, restored for the
+// field) and the 1/|r_p| factor of "scale".
+__device__ inline void store_split(const Geom& g, int x, int y, double re, double im, void* out,
+                                   int kind, size_t idx)
+{
+    if (kind == OUT_INTENSITY && !g.scale) {
+        static_cast<float*>(out)[idx] = (float)(re * re + im * im);
+        return;
+    }
+    const Frame f = pixel_frame(g, (double)x, (double)y);
+    if (g.scale) {
+        const double iR = 1.0 / f.R;
+        re *= iR;
+        im *= iR;
+    }
+    if (kind == OUT_INTENSITY) {
+        static_cast<float*>(out)[idx] = (float)(re * re + im * im);
+        return;
+    }
+    if (g.nf) {
+        double ph = g.krev * f.RmZ;
+        ph -= rint(ph);
+        double sn, cs;
+        sincos(kTwoPi * ph, &sn, &cs);
+        const double r2 = re * cs - im * sn;
+        const double i2 = re * sn + im * cs;
+        re = r2;
+        im = i2;
+    }
+    static_cast<float2*>(out)[idx] = make_float2((float)re, (float)im);
+}
+
+__device__ inline void store_fp64(int kind, double re, double im, void* out, size_t idx)
+{
+    if (kind == OUT_INTENSITY)
+        static_cast<double*>(out)[idx] = re * re + im * im;
+    else
+        static_cast<double2*>(out)[idx] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// FP32-split tile: accumulate emitters [e0, e1) for this lane's pixels.
+template <int PX, int PY, bool HAS_W, bool HAS_AMP, int SUM>
+__device__ __forceinline__ void split_tile(const Geom& g, const double* __restrict__ emit,
+                                           const double* __restrict__ amp, int64_t e0, int64_t e1,
+                                           int tx0, int ty0, float4* s0, float2* s1,
+                                           float (&acc_re)[PX * PY], float (&acc_im)[PX * PY],
+                                           float (&cmp_re)[PX * PY], float (&cmp_im)[PX * PY])
+{
+    constexpr int P = PX * PY;
+    constexpr int TX = 16 * PX, TY = 16 * PY;
+    const int tid = threadIdx.x;
+    const int lx = tid >> 4, ly = tid & 15;
+
+    double ccx, ccy;
+    tile_centre(g, tx0, ty0, TX, TY, ccx, ccy);
+    const Frame c = pixel_frame(g, ccx, ccy);
+
+    float dqx[P], dqy[P], dqz[P], lam[P];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+#pragma unroll
+        for (int j = 0; j < PY; ++j) {
+            const int p = i * PY + j;
+            const Frame f = pixel_frame(g, (double)(tx0 + lx * PX + i), (double)(ty0 + ly * PY + j));
+            dqx[p] = (float)(-g.krev * (f.u[0] - c.u[0]));
+            dqy[p] = (float)(-g.krev * (f.u[1] - c.u[1]));
+            dqz[p] = (float)(-g.krev * (f.u[2] - c.u[2]));
+            lam[p] = (float)(c.R / f.R);
+            acc_re[p] = 0.f;
+            acc_im[p] = 0.f;
+            cmp_re[p] = 0.f;
+            cmp_im[p] = 0.f;
+        }
+    }
+
+    for (int64_t base = e0; base < e1; base += kChunk) {
+        const int cnt = (e1 - base) < (int64_t)kChunk ? (int)(e1 - base) : kChunk;
+        __syncthreads();  // the previous chunk has been consumed by every lane
+        if (tid < cnt) {
+            const int64_t e = base + tid;
+            const double* a4 = emit + 4 * e;
+            const double ea = (HAS_AMP && amp != nullptr) ? amp[e] : 1.0;
+            const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], ea);
+            s0[tid] = make_float4(r.x, r.y, r.z, r.A);
+            s1[tid] = make_float2(r.W, r.amp);
+        }
+        __syncthreads();
+
+        float sre[P], sim[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            sre[p] = 0.f;
+            sim[p] = 0.f;
+        }
+#pragma unroll 2
+        for (int e = 0; e < cnt; ++e) {
+            const float4 r0 = s0[e];
+            float2 r1 = make_float2(0.f, 1.f);
+            if (HAS_W || HAS_AMP) r1 = s1[e];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                float t = HAS_W ? fmaf(lam[p], r1.x, r0.w) : r0.w;
+                t = fmaf(dqx[p], r0.x, t);
+                t = fmaf(dqy[p], r0.y, t);
+                t = fmaf(dqz[p], r0.z, t);
+                const float sn = __builtin_amdgcn_sinf(t);
+                const float cs = __builtin_amdgcn_cosf(t);
+                if (SUM == 0) {
+                    if (HAS_AMP) {
+                        sre[p] = fmaf(r1.y, cs, sre[p]);
+                        sim[p] = fmaf(r1.y, sn, sim[p]);
+                    } else {
+                        sre[p] += cs;
+                        sim[p] += sn;
+                    }
+                } else {
+                    kahan_add(acc_re[p], cmp_re[p], HAS_AMP ? r1.y * cs : cs);
+                    kahan_add(acc_im[p], cmp_im[p], HAS_AMP ? r1.y * sn : sn);
+                }
+            }
+        }
+        if (SUM == 0) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                kahan_add(acc_re[p], cmp_re[p], sre[p]);
+                kahan_add(acc_im[p], cmp_im[p], sim[p]);
+            }
+        }
+    }
+}
+
+template <int PX, int PY, bool HAS_W, bool HAS_AMP, int SUM>
+__global__ __launch_bounds__(kBlock) void k_fwd_split(Geom g, const double* __restrict__ emit,
+                                                      const double* __restrict__ amp, int64_t n,
+                                                      int tiles_y, int64_t per_split,
+                                                      float4* __restrict__ partial, void* out, int kind)
+{
+    constexpr int P = PX * PY;
+    constexpr int TX = 16 * PX, TY = 16 * PY;
+    __shared__ float4 s0[kChunk];
+    __shared__ float2 s1[kChunk];
+    const int tile = blockIdx.x;
+    const int split = blockIdx.y;
+    const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
+    const int64_t e0 = (int64_t)split * per_split;
+    const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
+    float re[P], im[P], cre[P], cim[P];
+    split_tile<PX, PY, HAS_W, HAS_AMP, SUM>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, re, im, cre, cim);
+
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+#pragma unroll
+        for (int j = 0; j < PY; ++j) {
+            const int p = i * PY + j;
+            const int x = tx0 + lx * PX + i, y = ty0 + ly * PY + j;
+            if (x < g.n0 && y < g.n1) {
+                const size_t idx = (size_t)x * g.n1 + y;
+                if (partial != nullptr)
+                    partial[(size_t)split * npix + idx] = make_float4(re[p], im[p], cre[p], cim[p]);
+                else
+                    store_split(g, x, y, (double)re[p] - (double)cre[p], (double)im[p] - (double)cim[p],
+                                out, kind, idx);
+            }
+        }
+    }
+}
+
+// Persistent batched kernel: workgroups pull (shot, tile) items from a queue.
+// Items are sorted by emitter count (largest first) on the host, so the tail
+// of the queue is made of short items.  Every workgroup leaves the loop once
+// the counter passes nitems.
+template <int PX, int PY, bool HAS_W, bool HAS_AMP, int SUM>
+__global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* __restrict__ emit,
+                                                          const double* __restrict__ amp,
+                                                          const WorkItem* __restrict__ items, int nitems,
+                                                          int* __restrict__ counter, int tiles_y, void* out,
+                                                          int kind)
+{
+    constexpr int P = PX * PY;
+    constexpr int TX = 16 * PX, TY = 16 * PY;
+    __shared__ float4 s0[kChunk];
+    __shared__ float2 s1[kChunk];
+    __shared__ int s_item;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nitems) break;
+        const WorkItem w = items[it];
+        const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
+        float re[P], im[P], cre[P], cim[P];
+        split_tile<PX, PY, HAS_W, HAS_AMP, SUM>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, re, im, cre,
+                                                cim);
+        char* outs = static_cast<char*>(out) +
+                     (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
+#pragma unroll
+        for (int i = 0; i < PX; ++i) {
+#pragma unroll
+            for (int j = 0; j < PY; ++j) {
+                const int p = i * PY + j;
+                const int x = tx0 + lx * PX + i, y = ty0 + ly * PY + j;
+                if (x < g.n0 && y < g.n1)
+                    store_split(g, x, y, (double)re[p] - (double)cre[p], (double)im[p] - (double)cim[p],
+                                outs, kind, (size_t)x * g.n1 + y);
+            }
+        }
+    }
+}
+
+__global__ void k_finalize_split(Geom g, const float4* __restrict__ partial, int splits, void* out,
+                                 int kind)
+{
+    const size_t npix = (size_t)g.n0 * g.n1;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= npix) return;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < splits; ++s) {
+        const float4 v = partial[(size_t)s * npix + idx];
+        re += (double)v.x - (double)v.z;
+        im += (double)v.y - (double)v.w;
+    }
+    store_split(g, (int)(idx / g.n1), (int)(idx % g.n1), re, im, out, kind, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Literal FP64 kernel (cpu.py:42-105 / sim.cu:43-139, FP64 geometry).
+template <int PX, int PY>
+__device__ __forceinline__ void fp64_tile(const Geom& g, const double* __restrict__ emit,
+                                          const double* __restrict__ amp, int64_t e0, int64_t e1,
+                                          int tx0, int ty0, double4* s0, double* s1,
+                                          double (&acc_re)[PX * PY], double (&acc_im)[PX * PY],
+                                          double (&cmp_re)[PX * PY], double (&cmp_im)[PX * PY])
+{
+    constexpr int P = PX * PY;
+    const int tid = threadIdx.x;
+    const int lx = tid >> 4, ly = tid & 15;
+    double dx[P], dy[P], dz[P], qx[P], qy[P], qz[P], rd[P], c2[P];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+#pragma unroll
+        for (int j = 0; j < PY; ++j) {
+            const int p = i * PY + j;
+            double r[3];
+            pixel_pos(g, (double)(tx0 + lx * PX + i), (double)(ty0 + ly * PY + j), r);
+            dx[p] = r[0];
+            dy[p] = r[1];
+            dz[p] = r[2];
+            // cpu.py:85-89 / sim.cu:77-84
+            rd[p] = 1.0 / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            qz[p] = g.k * (r[2] * rd[p] - 1.0);
+            qx[p] = g.k * (r[0] * rd[p]);
+            qy[p] = g.k * (r[1] * rd[p]);
+            c2[p] = g.k * rd[p] / 2.0;
+            acc_re[p] = 0.0;
+            acc_im[p] = 0.0;
+            cmp_re[p] = 0.0;
+            cmp_im[p] = 0.0;
+        }
+    }
+    for (int64_t base = e0; base < e1; base += kChunk) {
+        const int cnt = (e1 - base) < (int64_t)kChunk ? (int)(e1 - base) : kChunk;
+        __syncthreads();
+        if (tid < cnt) {
+            const int64_t e = base + tid;
+            const double* a4 = emit + 4 * e;
+            s0[tid] = make_double4(a4[0], a4[1], a4[2], a4[3]);
+            s1[tid] = (amp != nullptr) ? amp[e] : 1.0;
+        }
+        __syncthreads();
+        for (int e = 0; e < cnt; ++e) {
+            const double4 at = s0[e];
+            const double a = s1[e];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                double phase, mag;
+                if (g.nf) {
+                    const double ddx = dx[p] - at.x, ddy = dy[p] - at.y, ddz = dz[p] - at.z;
+                    const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                    phase = (dist - dz[p] + at.z) * g.k + at.w;
+                    mag = g.scale ? a / dist : a;
+                } else {
+                    phase = -(at.x * qx[p] + at.y * qy[p] + at.z * qz[p]) + at.w;
+                    if (g.secondorder) phase += c2[p] * (at.x * at.x + at.y * at.y);
+                    mag = g.scale ? a * rd[p] : a;
+                }
+                double sn, cs;
+                sincos(phase, &sn, &cs);
+                kahan_add_d(acc_re[p], cmp_re[p], mag * cs);
+                kahan_add_d(acc_im[p], cmp_im[p], mag * sn);
+            }
+        }
+    }
+}
+
+template <int PX, int PY>
+__global__ __launch_bounds__(kBlock) void k_fwd_fp64(Geom g, const double* __restrict__ emit,
+                                                     const double* __restrict__ amp, int64_t n, int tiles_y,
+                                                     int64_t per_split, double4* __restrict__ partial,
+                                                     void* out, int kind)
+{
+    constexpr int P = PX * PY;
+    constexpr int TX = 16 * PX, TY = 16 * PY;
+    __shared__ double4 s0[kChunk];
+    __shared__ double s1[kChunk];
+    const int tile = blockIdx.x, split = blockIdx.y;
+    const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
+    const int64_t e0 = (int64_t)split * per_split;
+    const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
+    double re[P], im[P], cre[P], cim[P];
+    fp64_tile<PX, PY>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, re, im, cre, cim);
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+#pragma unroll
+        for (int j = 0; j < PY; ++j) {
+            const int p = i * PY + j;
+            const int x = tx0 + lx * PX + i, y = ty0 + ly * PY + j;
+            if (x < g.n0 && y < g.n1) {
+                const size_t idx = (size_t)x * g.n1 + y;
+                if (partial != nullptr)
+                    partial[(size_t)split * npix + idx] = make_double4(re[p], im[p], cre[p], cim[p]);
+                else
+                    store_fp64(kind, re[p] - cre[p], im[p] - cim[p], out, idx);
+            }
+        }
+    }
+}
+
+template <int PX, int PY>
+__global__ __launch_bounds__(kBlock) void k_batched_fp64(Geom g, const double* __restrict__ emit,
+                                                         const double* __restrict__ amp,
+                                                         const WorkItem* __restrict__ items, int nitems,
+                                                         int* __restrict__ counter, int tiles_y, void* out,
+                                                         int kind)
+{
+    constexpr int P = PX * PY;
+    constexpr int TX = 16 * PX, TY = 16 * PY;
+    __shared__ double4 s0[kChunk];
+    __shared__ double s1[kChunk];
+    __shared__ int s_item;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nitems) break;
+        const WorkItem w = items[it];
+        const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
+        double re[P], im[P], cre[P], cim[P];
+        fp64_tile<PX, PY>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, re, im, cre, cim);
+        char* outs = static_cast<char*>(out) +
+                     (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(double) : sizeof(double2));
+#pragma unroll
+        for (int i = 0; i < PX; ++i) {
+#pragma unroll
+            for (int j = 0; j < PY; ++j) {
+                const int p = i * PY + j;
+                const int x = tx0 + lx * PX + i, y = ty0 + ly * PY + j;
+                if (x < g.n0 && y < g.n1)
+                    store_fp64(kind, re[p] - cre[p], im[p] - cim[p], outs, (size_t)x * g.n1 + y);
+            }
+        }
+    }
+}
+
+// kind bit 1 set: write complex64 / float32 (FP64 fallback of an FP32-split handle)
+__global__ void k_finalize_fp64(Geom g, const double4* __restrict__ partial, int splits, void* out,
+                                int kind)
+{
+    const size_t npix = (size_t)g.n0 * g.n1;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= npix) return;
+    double re = 0.0, im = 0.0, cre = 0.0, cim = 0.0;
+    for (int s = 0; s < splits; ++s) {
+        const double4 v = partial[(size_t)s * npix + idx];
+        kahan_add_d(re, cre, v.x - v.z);
+        kahan_add_d(im, cim, v.y - v.w);
+    }
+    re -= cre;
+    im -= cim;
+    if (kind & 2) {
+        if ((kind & 1) == OUT_INTENSITY)
+            static_cast<float*>(out)[idx] = (float)(re * re + im * im);
+        else
+            static_cast<float2*>(out)[idx] = make_float2((float)re, (float)im);
+    } else {
+        store_fp64(kind, re, im, out, idx);
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_reset_counter(int* counter) { *counter = 0; }
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// max_j |r_j|^2 as the bit pattern of a non-negative double (ordered like u64)
+__global__ __launch_bounds__(kBlock) void k_extent(const double* __restrict__ emit, int64_t n,
+                                                   unsigned long long* out)
+{
+    __shared__ double red[kBlock];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double* a = emit + 4 * i;
+        m = fmax(m, a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+template <int V, bool W, bool A, int S>
+static hipError_t launch_split_v(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
+                                 float4* partial, void* out, int kind, hipStream_t st)
+{
+    constexpr int PX = tile_px(V), PY = tile_py(V);
+    const int tiles_x = (g.n0 + 16 * PX - 1) / (16 * PX);
+    const int tiles_y = (g.n1 + 16 * PY - 1) / (16 * PY);
+    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    dim3 grid(tiles_x * tiles_y, splits);
+    hipLaunchKernelGGL((k_fwd_split<PX, PY, W, A, S>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
+                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+    return hipGetLastError();
+}
+
+template <int V, bool W, bool A, int S>
+static hipError_t launch_batched_split_v(const Geom& g, const double* emit, const double* amp,
+                                         const WorkItem* items, int nitems, int* counter, int nblocks,
+                                         void* out, int kind, hipStream_t st)
+{
+    constexpr int PX = tile_px(V), PY = tile_py(V);
+    const int tiles_y = (g.n1 + 16 * PY - 1) / (16 * PY);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL((k_batched_split<PX, PY, W, A, S>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp,
+                       items, nitems, counter, tiles_y, out, kind);
+    return hipGetLastError();
+}
+
+template <int V, bool W, bool A, int S>
+static int occupancy_split_v()
+{
+    constexpr int PX = tile_px(V), PY = tile_py(V);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_split<PX, PY, W, A, S>, kBlock, 0) !=
+        hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+#define IDISIM_DISPATCH_SPLIT(FN, ...)                                                        \
+    switch ((v << 3) | ((int)w << 2) | ((int)a << 1) | s) {                                   \
+        case 0x00: return FN<0, false, false, 0>(__VA_ARGS__);                                \
+        case 0x01: return FN<0, false, false, 1>(__VA_ARGS__);                                \
+        case 0x02: return FN<0, false, true, 0>(__VA_ARGS__);                                 \
+        case 0x03: return FN<0, false, true, 1>(__VA_ARGS__);                                 \
+        case 0x04: return FN<0, true, false, 0>(__VA_ARGS__);                                 \
+        case 0x05: return FN<0, true, false, 1>(__VA_ARGS__);                                 \
+        case 0x06: return FN<0, true, true, 0>(__VA_ARGS__);                                  \
+        case 0x07: return FN<0, true, true, 1>(__VA_ARGS__);                                  \
+        case 0x08: return FN<1, false, false, 0>(__VA_ARGS__);                                \
+        case 0x09: return FN<1, false, false, 1>(__VA_ARGS__);                                \
+        case 0x0a: return FN<1, false, true, 0>(__VA_ARGS__);                                 \
+        case 0x0b: return FN<1, false, true, 1>(__VA_ARGS__);                                 \
+        case 0x0c: return FN<1, true, false, 0>(__VA_ARGS__);                                 \
+        case 0x0d: return FN<1, true, false, 1>(__VA_ARGS__);                                 \
+        case 0x0e: return FN<1, true, true, 0>(__VA_ARGS__);                                  \
+        case 0x0f: return FN<1, true, true, 1>(__VA_ARGS__);                                  \
+        case 0x10: return FN<2, false, false, 0>(__VA_ARGS__);                                \
+        case 0x11: return FN<2, false, false, 1>(__VA_ARGS__);                                \
+        case 0x12: return FN<2, false, true, 0>(__VA_ARGS__);                                 \
+        case 0x13: return FN<2, false, true, 1>(__VA_ARGS__);                                 \
+        case 0x14: return FN<2, true, false, 0>(__VA_ARGS__);                                 \
+        case 0x15: return FN<2, true, false, 1>(__VA_ARGS__);                                 \
+        case 0x16: return FN<2, true, true, 0>(__VA_ARGS__);                                  \
+        case 0x17: return FN<2, true, true, 1>(__VA_ARGS__);                                  \
+        case 0x18: return FN<3, false, false, 0>(__VA_ARGS__);                                \
+        case 0x19: return FN<3, false, false, 1>(__VA_ARGS__);                                \
+        case 0x1a: return FN<3, false, true, 0>(__VA_ARGS__);                                 \
+        case 0x1b: return FN<3, false, true, 1>(__VA_ARGS__);                                 \
+        case 0x1c: return FN<3, true, false, 0>(__VA_ARGS__);                                 \
+        case 0x1d: return FN<3, true, false, 1>(__VA_ARGS__);                                 \
+        case 0x1e: return FN<3, true, true, 0>(__VA_ARGS__);                                  \
+        case 0x1f: return FN<3, true, true, 1>(__VA_ARGS__);                                  \
+        default: break;                                                                       \
+    }
+
+hipError_t launch_fwd_split(int v, bool w, bool a, int s, const Geom& g, const double* emit, const double* amp,
+                            int64_t n, int splits, float4* partial, void* out, int kind, hipStream_t st)
+{
+    IDISIM_DISPATCH_SPLIT(launch_split_v, g, emit, amp, n, splits, partial, out, kind, st)
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_batched_split(int v, bool w, bool a, int s, const Geom& g, const double* emit,
+                                const double* amp, const WorkItem* items, int nitems, int* counter, int nblocks,
+                                void* out, int kind, hipStream_t st)
+{
+    IDISIM_DISPATCH_SPLIT(launch_batched_split_v, g, emit, amp, items, nitems, counter, nblocks, out, kind, st)
+    return hipErrorInvalidValue;
+}
+
+int occupancy_batched_split(int v, bool w, bool a, int s)
+{
+    IDISIM_DISPATCH_SPLIT(occupancy_split_v)
+    return 1;
+}
+
+hipError_t launch_finalize_split(const Geom& g, const float4* partial, int splits, void* out, int kind,
+                                 hipStream_t st)
+{
+    const size_t npix = (size_t)g.n0 * g.n1;
+    hipLaunchKernelGGL(k_finalize_split, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, g, partial,
+                       splits, out, kind);
+    return hipGetLastError();
+}
+
+template <int V>
+static hipError_t launch_fp64_v(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
+                                double4* partial, void* out, int kind, hipStream_t st)
+{
+    constexpr int PX = tile_px(V), PY = tile_py(V);
+    const int tiles_x = (g.n0 + 16 * PX - 1) / (16 * PX);
+    const int tiles_y = (g.n1 + 16 * PY - 1) / (16 * PY);
+    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    dim3 grid(tiles_x * tiles_y, splits);
+    hipLaunchKernelGGL((k_fwd_fp64<PX, PY>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
+                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+    return hipGetLastError();
+}
+
+template <int V>
+static hipError_t launch_batched_fp64_v(const Geom& g, const double* emit, const double* amp,
+                                        const WorkItem* items, int nitems, int* counter, int nblocks, void* out,
+                                        int kind, hipStream_t st)
+{
+    constexpr int PX = tile_px(V), PY = tile_py(V);
+    const int tiles_y = (g.n1 + 16 * PY - 1) / (16 * PY);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL((k_batched_fp64<PX, PY>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp, items,
+                       nitems, counter, tiles_y, out, kind);
+    return hipGetLastError();
+}
+
+template <int V>
+static int occupancy_fp64_v()
+{
+    constexpr int PX = tile_px(V), PY = tile_py(V);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_fp64<PX, PY>, kBlock, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_fwd_fp64(int v, const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
+                           double4* partial, void* out, int kind, hipStream_t st)
+{
+    switch (v) {
+        case 0: return launch_fp64_v<0>(g, emit, amp, n, splits, partial, out, kind, st);
+        case 1: return launch_fp64_v<1>(g, emit, amp, n, splits, partial, out, kind, st);
+        case 2: return launch_fp64_v<2>(g, emit, amp, n, splits, partial, out, kind, st);
+        case 3: return launch_fp64_v<3>(g, emit, amp, n, splits, partial, out, kind, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batched_fp64(int v, const Geom& g, const double* emit, const double* amp,
+                               const WorkItem* items, int nitems, int* counter, int nblocks, void* out, int kind,
+                               hipStream_t st)
+{
+    switch (v) {
+        case 0: return launch_batched_fp64_v<0>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        case 1: return launch_batched_fp64_v<1>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        case 2: return launch_batched_fp64_v<2>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        case 3: return launch_batched_fp64_v<3>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int occupancy_batched_fp64(int v)
+{
+    switch (v) {
+        case 0: return occupancy_fp64_v<0>();
+        case 1: return occupancy_fp64_v<1>();
+        case 2: return occupancy_fp64_v<2>();
+        case 3: return occupancy_fp64_v<3>();
+        default: return 1;
+    }
+}
+
+hipError_t launch_finalize_fp64(const Geom& g, const double4* partial, int splits, void* out, int kind,
+                                hipStream_t st)
+{
+    const size_t npix = (size_t)g.n0 * g.n1;
+    hipLaunchKernelGGL(k_finalize_fp64, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, g, partial,
+                       splits, out, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, out, 0ull);
+    int64_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_extent, dim3((unsigned)blocks), dim3(kBlock), 0, st, emit, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace idisim

@@ -1,0 +1,26 @@
+"""Direct-sum wavefield simulation on MI355X: drop-in for ``idi.simulation``.
+
+    from idi_amd import simulation as sim
+    obj = sim.simobj.sphere(E=8000, N=1000, r=0.05)
+    fields = sim.auto.simulate(4, obj, 128, 75, 1e5, settings="single nf")
+
+``hip`` is the backend (simulate, simulate_gen, simulate_batched) and ``auto``
+refers to it, as ``idi.simulation.auto`` refers to the CUDA backend when one is
+available (idi/simulation/__init__.py:9-31).  Unlike the reference there is no
+CPU fallback: on a machine without a ROCm GPU the calls raise RuntimeError.
+``autograd`` holds the torch.autograd.Function with the hand-written adjoint.
+"""
+from . import hip, simobj  # noqa: F401
+from .common import randomphotons  # noqa: F401
+
+auto = hip
+
+__all__ = ["hip", "simobj", "auto", "autograd", "randomphotons"]
+
+
+def __getattr__(name):
+    if name == "autograd":
+        from . import autograd
+
+        return autograd
+    raise AttributeError(name)

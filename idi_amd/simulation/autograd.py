@@ -1,0 +1,54 @@
+"""torch.autograd.Function around the FP32-split forward and the hand-written adjoint kernel.
+
+    I = intensity(pos, phase, amp=None, Ndet=..., pixelsize=..., detz=..., k=..., settings="single nf")
+    loss = ((I - I_target) ** 2).sum(); loss.backward()
+
+pos (N, 3) and phase (N,) (and optionally amp (N,)) are CUDA tensors; the
+returned intensity is float32 (N0, N1).  The backward pass calls
+idisim_adjoint, which computes dL/dpos, dL/dphase and dL/damp from dL/dI and
+the saved complex64 field (no reference counterpart).  The 1/r "scale" model
+has no adjoint.
+"""
+from __future__ import annotations
+
+import torch
+
+from .. import _lib
+from .hip import get_handle
+
+
+class _WavefieldIntensity(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, phase, amp, handle, extent):
+        emit = torch.cat([pos.detach().to(torch.float64), phase.detach().to(torch.float64).reshape(-1, 1)],
+                         1).contiguous()
+        a = None if amp is None else amp.detach().to(torch.float64).contiguous()
+        field = handle.forward(emit, a, None, _lib.OUT_FIELD, extent)
+        ctx.handle = handle
+        ctx.extent = extent
+        ctx.has_amp = amp is not None
+        ctx.dtypes = (pos.dtype, phase.dtype, None if amp is None else amp.dtype)
+        ctx.save_for_backward(emit, field, a if a is not None else torch.empty(0, device=pos.device))
+        return field.real * field.real + field.imag * field.imag
+
+    @staticmethod
+    def backward(ctx, gI):
+        emit, field, a = ctx.saved_tensors
+        amp = a if ctx.has_amp else None
+        gpos, gph, gamp = ctx.handle.adjoint(emit, field, gI.to(torch.float32).contiguous(), amp, ctx.extent,
+                                             want_amp=ctx.has_amp)
+        pd, phd, ad = ctx.dtypes
+        return (gpos.to(pd), gph.to(phd).reshape(-1), None if gamp is None else gamp.to(ad), None, None)
+
+
+def intensity(pos, phase, amp=None, *, Ndet, pixelsize, detz, k, settings="single nf", detangles=(0, 0),
+              detoffset=(0, 0), extent=None):
+    """|E|^2 on the detector, differentiable w.r.t. pos, phase and amp (FP32-split)."""
+    if "single" not in settings:
+        settings = settings + " single"
+    if "scale" in settings:
+        raise NotImplementedError("the 1/r 'scale' model has no adjoint")
+    h = get_handle(Ndet, pixelsize, detz, k, settings, detangles, detoffset, device=pos.device.index)
+    if extent is None:
+        extent = float(torch.linalg.vector_norm(pos.detach().to(torch.float64), dim=1).max()) if len(pos) else 0.0
+    return _WavefieldIntensity.apply(pos, phase, amp, h, float(extent))

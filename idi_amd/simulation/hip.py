@@ -1,0 +1,243 @@
+"""HIP backend of the direct-sum wavefield: drop-in for idi/simulation/cuda.py (and cpu.py).
+
+    simulate(Nimg, simobject, Ndet, pixelsize, detz, k=None, settings="double", verbose=True,
+             detangles=(0, 0), detoffset=(0, 0), returngpu=False, *args, **kwargs)
+    simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", maximg=inf,
+                 detangles=(0, 0), detoffset=(0, 0), returngpu=False, *args, **kwargs)
+
+have the signatures and return types of cuda.py:141-202 / cuda.py:16-138:
+``simulate`` returns complex128 (Nimg, N0, N1) on the host; ``simulate_gen``
+yields one (N0, N1) field per image, complex64 for "single" and complex128
+otherwise, or a device tensor with ``returngpu=True``.
+
+``settings`` is the reference's substring string (cuda.py:38-74); unknown
+tokens are ignored:
+    single       FP32-split kernels: the large phase is formed in FP64 and
+                 range-reduced, sin/cos and sums run in FP32 (complex64 out)
+    double       (default) literal FP64 kernels (complex128 out)
+    nf           near field: exact distance |r_p - r_j|
+    scale        1/r amplitude
+    secondorder  far-field Fresnel term k/(2|r_p|)(x^2 + y^2)
+    nofast       accepted, no effect (the kernels never use fast-math)
+
+Extensions (keyword arguments; the reference signatures are unchanged):
+    amplitudes   (N,) per-emitter amplitudes, or a callable returning them per
+                 image; default: ``simobject.get_amplitudes()`` if present
+    intensity    True: return |E|^2 (float) instead of the field
+    simulate_batched(...)  many shots with uneven emitter counts in one
+                 persistent, graph-replayed launch.
+
+There is no CPU path: without a ROCm GPU and the built library every call
+raises RuntimeError.
+"""
+from __future__ import annotations
+
+import numpy as _np
+
+from .. import _lib
+from ..util import rotation as _rotation
+
+__all__ = ["simulate", "simulate_gen", "simulate_batched", "get_handle", "parse_settings"]
+
+_HANDLES = {}
+_MAX_HANDLES = 8
+
+
+def parse_settings(settings):
+    """settings substrings -> dict of flags (cuda.py:53-74)."""
+    s = settings or ""
+    return {
+        "single": "single" in s,
+        "nf": "nf" in s,
+        "scale": "scale" in s,
+        "secondorder": "secondorder" in s,
+    }
+
+
+def _ndet(Ndet):
+    if _np.size(Ndet) == 1:
+        n = int(_np.asarray(Ndet).ravel()[0])
+        return n, n
+    return int(Ndet[0]), int(Ndet[1])
+
+
+def get_handle(Ndet, pixelsize, detz, k, settings="double", detangles=(0, 0), detoffset=(0, 0), device=None):
+    """Cached library handle for one geometry / precision (created on first use)."""
+    import torch
+
+    n0, n1 = _ndet(Ndet)
+    f = parse_settings(settings)
+    rot = _rotation(detangles[0], detangles[1], 0) if _np.any(detangles) else None
+    dev = torch.cuda.current_device() if device is None else int(device)
+    key = (n0, n1, float(pixelsize), float(detz), float(k), tuple(float(v) for v in detoffset),
+           None if rot is None else tuple(rot.ravel()), f["nf"], f["scale"], f["secondorder"], f["single"], dev)
+    h = _HANDLES.get(key)
+    if h is None:
+        if len(_HANDLES) >= _MAX_HANDLES:
+            _HANDLES.pop(next(iter(_HANDLES))).close()
+        h = _lib.Handle(n0, n1, pixelsize, detz, k, detoffset=detoffset, rot=rot, nf=f["nf"], scale=f["scale"],
+                        secondorder=f["secondorder"],
+                        precision=_lib.PREC_FP32SPLIT if f["single"] else _lib.PREC_FP64, device=dev)
+        _HANDLES[key] = h
+    return h
+
+
+def _amplitudes(simobject, amplitudes):
+    if amplitudes is not None:
+        a = amplitudes() if callable(amplitudes) else amplitudes
+    elif hasattr(simobject, "get_amplitudes"):
+        a = simobject.get_amplitudes()
+    else:
+        a = None
+    return None if a is None else _np.ascontiguousarray(a, dtype=_np.float64).reshape(-1)
+
+
+def _extent(atoms):
+    return float(_np.sqrt(_np.max(_np.einsum("ij,ij->i", atoms[:, :3], atoms[:, :3])))) if len(atoms) else 0.0
+
+
+class _Staging:
+    """Pinned host buffer + device buffer for one image's emitters."""
+
+    def __init__(self, n, with_amp, device):
+        import torch
+
+        self.h = torch.empty((n, 4), dtype=torch.float64, pin_memory=True)
+        self.d = torch.empty((n, 4), dtype=torch.float64, device=device)
+        self.ha = torch.empty((n,), dtype=torch.float64, pin_memory=True) if with_amp else None
+        self.da = torch.empty((n,), dtype=torch.float64, device=device) if with_amp else None
+        self.ready = None  # event after the H2D copy; the host buffer may be rewritten once it has passed
+
+    def upload(self, atoms, amp):
+        import torch
+
+        if self.ready is not None:
+            self.ready.synchronize()
+        self.h.numpy()[...] = atoms
+        if amp is not None:
+            self.ha.numpy()[...] = amp
+        self.d.copy_(self.h, non_blocking=True)
+        if amp is not None:
+            self.da.copy_(self.ha, non_blocking=True)
+        self.ready = torch.cuda.Event()
+        self.ready.record()
+
+
+def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", maximg=_np.inf, detangles=(0, 0),
+                 detoffset=(0, 0), returngpu=False, *args, amplitudes=None, intensity=False, **kwargs):
+    """Generator of simulated wavefields (cuda.py:16-138).
+
+    Two-deep pipeline: while the GPU computes image i, the host draws the
+    emitters of image i+1 and copies them through pinned memory on the same
+    stream; image i is copied back (or cloned on the device) after image i+1
+    has been launched.
+    """
+    import torch
+
+    _lib._require_cuda()
+    if k is None:
+        k = simobject.k
+    n0, n1 = _ndet(Ndet)
+    h = get_handle((n0, n1), pixelsize, detz, k, settings, detangles, detoffset)
+    dev = torch.device("cuda", h.device)
+    kind = _lib.OUT_INTENSITY if intensity else _lib.OUT_FIELD
+    dtype = h.real_dtype if intensity else h.field_dtype
+
+    def _gen():
+        count = 0
+        if maximg <= 0:
+            return
+        with torch.cuda.device(dev):
+            a0 = _amplitudes(simobject, amplitudes)
+            stages = [_Staging(simobject.N, a0 is not None, dev) for _ in range(2)]
+            outs = [torch.empty((n0, n1), dtype=dtype, device=dev) for _ in range(2)]
+
+            def launch(slot, atoms, amp):
+                st = stages[slot]
+                st.upload(atoms, amp)
+                h.forward(st.d, st.da if amp is not None else None, outs[slot], kind, _extent(atoms))
+
+            launch(0, simobject.get(), a0)
+            slot = 0
+            while count < maximg:
+                count += 1
+                if count < maximg:
+                    launch(1 - slot, simobject.get(), _amplitudes(simobject, amplitudes))
+                if returngpu:
+                    ret = outs[slot].clone()
+                else:
+                    ret = outs[slot].cpu().numpy()
+                yield ret
+                slot = 1 - slot
+
+    return _gen()
+
+
+def simulate(Nimg, simobject, Ndet, pixelsize, detz, k=None, settings="double", verbose=True, detangles=(0, 0),
+             detoffset=(0, 0), returngpu=False, *args, amplitudes=None, intensity=False, **kwargs):
+    """Array of Nimg simulated wavefields (cuda.py:141-202): complex128 (Nimg, N0, N1) on the host,
+    or a device tensor with ``returngpu=True``.  Ctrl-C returns the images finished so far."""
+    import torch
+
+    n0, n1 = _ndet(Ndet)
+    gen = simulate_gen(simobject, (n0, n1), pixelsize, detz, k, settings=settings, maximg=Nimg, detangles=detangles,
+                       detoffset=detoffset, returngpu=returngpu, amplitudes=amplitudes, intensity=intensity)
+    if returngpu:
+        h = get_handle((n0, n1), pixelsize, detz, simobject.k if k is None else k, settings, detangles, detoffset)
+        dt = h.real_dtype if intensity else h.field_dtype
+        result = torch.empty((Nimg, n0, n1), dtype=dt, device=torch.device("cuda", h.device))
+    else:
+        result = _np.empty((Nimg, n0, n1), _np.float64 if intensity else _np.complex128)
+    i = -1
+    try:
+        for i, img in enumerate(gen):
+            if verbose:
+                print(i, end=". ", flush=True)
+            result[i, ...] = img
+        return result
+    except KeyboardInterrupt:
+        if verbose:
+            print("Interrupted")
+        return result[: max(i, 0)]
+
+
+def simulate_batched(emitters, Ndet, pixelsize, detz, k, settings="single", detangles=(0, 0), detoffset=(0, 0),
+                     amplitudes=None, intensity=False, returngpu=False, extent=None):
+    """Many shots with uneven emitter counts in one persistent launch (no reference counterpart).
+
+    emitters:   list of (N_s, 4) float64 arrays (x, y, z, phase), or one
+                concatenated array/tensor together with ``offsets`` given as
+                ``(array, offsets)``
+    amplitudes: None, a list of (N_s,) arrays, or one concatenated array
+    returns     (nshots, N0, N1) field (complex64 for "single") or intensity,
+                on the host (numpy) or as a device tensor with ``returngpu``.
+    Repeated calls with the same shot sizes replay a captured hipGraph.
+    """
+    import torch
+
+    _lib._require_cuda()
+    n0, n1 = _ndet(Ndet)
+    h = get_handle((n0, n1), pixelsize, detz, k, settings, detangles, detoffset)
+    dev = torch.device("cuda", h.device)
+    if isinstance(emitters, tuple):
+        cat, offsets = emitters
+        offsets = _np.asarray(offsets, dtype=_np.int64)
+    else:
+        sizes = [len(e) for e in emitters]
+        offsets = _np.concatenate([[0], _np.cumsum(sizes)]).astype(_np.int64)
+        cat = _np.concatenate([_np.asarray(e, dtype=_np.float64).reshape(-1, 4) for e in emitters])
+    if isinstance(cat, torch.Tensor):
+        d = cat.to(device=dev, dtype=torch.float64).contiguous()
+        ext = float(torch.linalg.vector_norm(d[:, :3], dim=1).max()) if extent is None and len(d) else extent
+    else:
+        cat = _np.ascontiguousarray(cat, dtype=_np.float64)
+        ext = _extent(cat) if extent is None else extent
+        d = torch.from_numpy(cat).to(dev)
+    da = None
+    if amplitudes is not None:
+        if isinstance(amplitudes, (list, tuple)):
+            amplitudes = _np.concatenate([_np.asarray(a, dtype=_np.float64).ravel() for a in amplitudes])
+        da = torch.as_tensor(amplitudes, dtype=torch.float64).to(dev).contiguous()
+    kind = _lib.OUT_INTENSITY if intensity else _lib.OUT_FIELD
+    out = h.forward_batched(d, offsets, da, None, kind, ext or 0.0)
+    return out if returngpu else out.cpu().numpy()

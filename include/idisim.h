@@ -1,0 +1,154 @@
+/*
+ * idisim -- MI355X (gfx950) direct-sum wavefield simulation, C ABI.
+ *
+ * For every detector pixel p the library evaluates
+ *     E_p = sum_j a_j m_pj exp(i theta_pj),   I_p = |E_p|^2
+ * with the phase models of idi's simulation package:
+ *   near field ("nf"):  theta = k (|r_p - r_j| - z_p + z_j) + phi_j
+ *                       (idi/simulation/cpu.py:42-55, sim.cu:43-69)
+ *   far field:          theta = -q_p . r_j + phi_j,  q_p = k (r_p/|r_p| - z_hat)
+ *                       (cpu.py:57-66,85-89, sim.cu:71-95), optional
+ *                       "secondorder" + k/(2|r_p|) (x_j^2 + y_j^2) (sim.cu:88)
+ *   m_pj = 1, or 1/|r_p - r_j| (nf) / 1/|r_p| (ff) with "scale"
+ *   (cpu.py:48-54,60-65).
+ * Pixel (x, y), x the row index: r_p = M ((x - n0/2) s + off0, (y - n1/2) s + off1, z)
+ * with true division (cpu.py:76-80) and M the optional detector rotation.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer unless its comment says host.
+ *     The caller owns all buffers; the library never frees caller memory.
+ *   - Emitters are the reference layout: n x 4 row-major float64 (x, y, z, phi),
+ *     as simobj.get() returns them (simobj.py:63-66, cpu.py:69).
+ *   - Return value 0 = success, < 0 = error; idisim_last_error() gives a
+ *     thread-local message.  The library never calls exit() or abort().
+ *   - A handle is used by one host thread / stream at a time; different
+ *     handles are independent.  Work is enqueued on `stream` (a hipStream_t;
+ *     NULL = the legacy default stream) and is asynchronous unless stated.
+ */
+#ifndef IDISIM_H
+#define IDISIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IDISIM_VERSION 1
+
+/* geometry flags (settings substrings of cuda.py:53-74 / cpu.py:189-190) */
+#define IDISIM_NF          1u  /* "nf": exact distance (near field)            */
+#define IDISIM_SCALE       2u  /* "scale": 1/r amplitude                       */
+#define IDISIM_SECONDORDER 4u  /* "secondorder": far-field Fresnel term       */
+#define IDISIM_ROTATE      8u  /* apply rot[] to the pixel positions          */
+
+/* precision of a handle */
+#define IDISIM_PREC_FP64      0 /* literal FP64 formula, complex128 output      */
+#define IDISIM_PREC_FP32SPLIT 1 /* FP64 range-reduced phase, FP32 sincos/sums,
+                                   complex64 output                            */
+
+/* output kinds */
+#define IDISIM_OUT_FIELD     0 /* complex (re, im) interleaved, n0 x n1        */
+#define IDISIM_OUT_INTENSITY 1 /* |E|^2, n0 x n1                               */
+
+/* error codes */
+#define IDISIM_OK            0
+#define IDISIM_E_INVALID    -1 /* invalid argument                             */
+#define IDISIM_E_HIP        -2 /* HIP runtime error                            */
+#define IDISIM_E_NOMEM      -3 /* device allocation failed                     */
+#define IDISIM_E_PRECISION  -4 /* FP32-split error bound not met (see options) */
+#define IDISIM_E_UNSUPPORTED -5
+
+/* options (idisim_set_option) */
+#define IDISIM_OPT_AUTOTUNE     1 /* 1 (default): time candidate plans on first use */
+#define IDISIM_OPT_TILE         2 /* force tile variant 0..3 (-1 = planner)         */
+#define IDISIM_OPT_SPLIT        3 /* force emitter split count (0 = planner)        */
+#define IDISIM_OPT_SUMMODE      4 /* 0 (default): per-chunk partials merged with
+                                     Kahan; 1: Kahan on every term (cpu.py:97-105) */
+#define IDISIM_OPT_GRAPH        5 /* 1 (default): batched calls replay a hipGraph   */
+#define IDISIM_OPT_TOL_PPB      6 /* per-term phase error bound, parts per 1e9 of a
+                                     revolution (default 3000 = 3e-6 rev)          */
+#define IDISIM_OPT_FALLBACK     7 /* 1 (default): FP32-split handles run the FP64
+                                     kernel when no tile meets the bound; 0: error */
+
+typedef struct idisim_handle idisim_handle;
+
+typedef struct idisim_geometry {
+    int64_t n0, n1;       /* detector pixels (rows, cols)                  */
+    double pixelsize;     /* pixel pitch, same length unit as emitters     */
+    double detz;          /* detector distance                             */
+    double k;             /* angular wavenumber (1 / length unit)          */
+    double detoffset[2];  /* detector centre offset (x, y)                 */
+    double rot[9];        /* row-major rotation (IDISIM_ROTATE)            */
+    uint32_t flags;       /* IDISIM_NF | IDISIM_SCALE | ...                */
+} idisim_geometry;
+
+/* Plan actually used by the last forward/adjoint call of a handle. */
+typedef struct idisim_plan {
+    int32_t precision;    /* precision that ran (FP64 after a fallback)    */
+    int32_t tile;         /* tile variant index                            */
+    int32_t tile_rows, tile_cols; /* pixels per workgroup tile             */
+    int32_t split;        /* emitter split count                           */
+    int32_t summode;
+    int32_t graph;        /* 1 if a hipGraph was replayed                  */
+    int32_t fallback;     /* 1 if FP32-split fell back to FP64             */
+    double phase_bound;   /* a-priori per-term phase error bound, rev      */
+    double tuned_ms;      /* best candidate time measured by the autotuner */
+} idisim_plan;
+
+/* Library ABI version (IDISIM_VERSION). */
+int idisim_version(void);
+
+/* Thread-local message of the last error ("" if none). */
+const char *idisim_last_error(void);
+
+/* Create a handle on `device` for a fixed geometry and precision.
+   Replaces the per-call kernel set-up of cuda.py:53-86 (flags, rotation,
+   module compile) and the closure of cpu.py:6-40 (get_kernel). */
+int idisim_create(idisim_handle **out, const idisim_geometry *geom, int precision, int device);
+
+int idisim_destroy(idisim_handle *h);
+
+int idisim_set_option(idisim_handle *h, int option, int64_t value);
+
+int idisim_get_plan(const idisim_handle *h, idisim_plan *out);
+
+/* One shot.  Replaces the kernels wfkernelf / wfkerneld (sim.cu:141-149) as
+   launched by cuda.py:91-107, and the numba kernel(ret, atom) of cpu.py:68-107.
+     emitters:   n x 4 float64 (x, y, z, phi)
+     amplitudes: n float64 or NULL (all ones; the reference has no amplitudes)
+     extent:     an upper bound of max_j |r_j| (used by the FP32-split error
+                 planner); <= 0 lets the library measure it (synchronises)
+     out:        complex64 (FP32SPLIT) / complex128 (FP64) n0 x n1 for FIELD,
+                 float32 / float64 n0 x n1 for INTENSITY, row-major, ret[x, y] */
+int idisim_forward(idisim_handle *h, const double *emitters, const double *amplitudes,
+                   int64_t n, double extent, void *out, int out_kind, void *stream);
+
+/* Many shots with uneven emitter counts in one persistent launch (replayed as
+   a hipGraph when the buffers and counts repeat).  No reference counterpart;
+   it batches the per-image loop of cuda.py:108-136 / cpu.py:207-212.
+     emitters:   concatenated (sum n_s) x 4 float64
+     offsets:    HOST array of nshots + 1 int64, offsets[0] = 0
+     out:        nshots x n0 x n1 of the output kind/type of idisim_forward */
+int idisim_forward_batched(idisim_handle *h, const double *emitters, const double *amplitudes,
+                           const int64_t *offsets, int64_t nshots, double extent,
+                           void *out, int out_kind, void *stream);
+
+/* Adjoint of I = |E|^2 for a loss L(I) (no reference counterpart).
+     field:      the complex64 field that idisim_forward returned for the same
+                 emitters (FP32SPLIT handles only; not with IDISIM_SCALE)
+     dLdI:       float32 n0 x n1
+     grad_pos:   float32 n x 3  dL/d(x, y, z)
+     grad_phase: float32 n      dL/dphi
+     grad_amp:   float32 n      dL/da (may be NULL)                        */
+int idisim_adjoint(idisim_handle *h, const double *emitters, const double *amplitudes,
+                   int64_t n, double extent, const void *field, const float *dLdI,
+                   float *grad_pos, float *grad_phase, float *grad_amp, void *stream);
+
+/* Largest |r_j| of n emitters (device reduction; synchronises `stream`). */
+int idisim_extent(const double *emitters, int64_t n, double *extent_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IDISIM_H */

@@ -1,0 +1,216 @@
+/*
+ * FP64 CPU restatement of idi's time-independent direct-sum wavefield.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests/, the smoke check
+ * and the cpu_baseline leg of bench.py call it to CHECK the HIP library.  The
+ * product path (idi_amd) never links, loads or calls it.
+ *
+ * Follows, line for line in meaning (the code is written from scratch):
+ *   idi/simulation/cpu.py:30-40    casts, rotation M = rotation(theta, phi, 0)
+ *   idi/simulation/cpu.py:42-55    near-field phase (nf), optional 1/dist scale
+ *   idi/simulation/cpu.py:57-66    far-field phase (ff), optional 1/|r_pix| scale
+ *   idi/simulation/cpu.py:74-80    row loop, pixel coordinates (true division),
+ *                                  rotation applied as M @ (detx, dety, detz)
+ *   idi/simulation/cpu.py:81-107   far-field q set-up, complex Kahan summation
+ *                                  in emitter order, complex store at ret[x, y]
+ *   idi/simulation/sim.cu:77-88    "secondorder" far-field term (GPU-only in
+ *                                  the reference): phase += k/(2|r_pix|)(x^2+y^2)
+ *   idi/util/__init__.py:348-372   rotation(alpha, beta, gamma) (built by the
+ *                                  Python side and passed in as M)
+ * Extensions with no reference counterpart (pinned by closed forms only):
+ *   per-emitter amplitudes a_j multiplying every term;
+ *   the analytic FP64 gradient of L(I) with respect to positions, phases and
+ *   amplitudes (oracle_adjoint).
+ *
+ * Compiled with -O3 -ffp-contract=fast and WITHOUT -ffast-math: the reference
+ * kernel is compiled by numba with fastmath={contract, arcp, ninf, nnan}
+ * (cpu.py:71), which allows FMA contraction but no reassociation, so the Kahan
+ * compensation survives.  glibc sin/cos/sqrt.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <omp.h>
+
+typedef struct {
+    int64_t n0, n1;        /* detector pixels along x (rows) and y (cols)   */
+    double pixelsize;      /* same length unit as the emitters (um)          */
+    double detz;           /* detector distance                              */
+    double k;              /* angular wavenumber (1/length)                  */
+    double off0, off1;     /* detoffset (x, y)                               */
+    double M[9];           /* row-major rotation, used when rot != 0         */
+    int32_t rot, nf, scale, secondorder;
+} oracle_geom;
+
+/* cpu.py:76-80 */
+static void pixel_position(const oracle_geom *g, int64_t x, int64_t y,
+                           double *px, double *py, double *pz)
+{
+    double detx = ((double)x - (double)g->n0 / 2.0) * g->pixelsize + g->off0;
+    double dety = ((double)y - (double)g->n1 / 2.0) * g->pixelsize + g->off1;
+    double detz = g->detz;
+    if (g->rot) {
+        const double *M = g->M;
+        double rx = M[0] * detx + M[1] * dety + M[2] * detz;
+        double ry = M[3] * detx + M[4] * dety + M[5] * detz;
+        double rz = M[6] * detx + M[7] * dety + M[8] * detz;
+        detx = rx; dety = ry; detz = rz;
+    }
+    *px = detx; *py = dety; *pz = detz;
+}
+
+/* Per-pixel far-field constants, cpu.py:85-89 and sim.cu:77-84. */
+typedef struct { double qx, qy, qz, rdist, c2; } ffconst;
+
+static ffconst ff_setup(const oracle_geom *g, double detx, double dety, double detz)
+{
+    ffconst f;
+    f.rdist = 1.0 / sqrt(detx * detx + dety * dety + detz * detz);
+    f.qz = g->k * (detz * f.rdist - 1.0);
+    f.qx = g->k * (detx * f.rdist);
+    f.qy = g->k * (dety * f.rdist);
+    f.c2 = g->k * f.rdist / 2.0;
+    return f;
+}
+
+/* One emitter-pixel term: phase, and the real amplitude multiplying e^{i phase}. */
+static inline void term(const oracle_geom *g, const double *atom, double amp,
+                        double detx, double dety, double detz, const ffconst *f,
+                        double *phase, double *mag, double *dist_out)
+{
+    if (g->nf) {
+        /* cpu.py:44-47 */
+        double dx = detx - atom[0], dy = dety - atom[1], dz = detz - atom[2];
+        double dist = sqrt(dx * dx + dy * dy + dz * dz);
+        *phase = (dist - detz + atom[2]) * g->k + atom[3];
+        *mag = g->scale ? amp * (1.0 / dist) : amp;        /* cpu.py:52-54 */
+        if (dist_out) *dist_out = dist;
+    } else {
+        /* cpu.py:59 */
+        double ph = -(atom[0] * f->qx + atom[1] * f->qy + atom[2] * f->qz) + atom[3];
+        if (g->secondorder) ph += f->c2 * (atom[0] * atom[0] + atom[1] * atom[1]);
+        *phase = ph;
+        *mag = g->scale ? amp * f->rdist : amp;             /* cpu.py:64-65 */
+        if (dist_out) *dist_out = 0.0;
+    }
+}
+
+/*
+ * Field at every pixel of the listed detector rows.
+ *   atoms: n x 4 row-major (x, y, z, phase), as simobj.get() returns it.
+ *   amp:   n amplitudes or NULL (all ones, the reference behaviour).
+ *   rows:  nrows row indices (x) to evaluate; NULL means rows 0..nrows-1.
+ *   out:   nrows x n1 x 2 doubles (re, im), row-major.
+ */
+void oracle_field_rows(const oracle_geom *g, const double *atoms, const double *amp,
+                       int64_t n, const int64_t *rows, int64_t nrows, double *out,
+                       int nthreads)
+{
+    const int64_t n1 = g->n1;
+    const int64_t npix = nrows * n1;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    /* rows are split over threads like numba's prange (cpu.py:74); pixels are
+       flattened so that a handful of sampled rows still fills every core */
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t pix = 0; pix < npix; ++pix) {
+        const int64_t r = pix / n1, y = pix % n1;
+        const int64_t x = rows ? rows[r] : r;
+        double detx, dety, detz;
+        pixel_position(g, x, y, &detx, &dety, &detz);
+        ffconst f = {0, 0, 0, 0, 0};
+        if (!g->nf) f = ff_setup(g, detx, dety, detz);
+        double accx = 0.0, accy = 0.0, cx = 0.0, cy = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            const double *atom = atoms + 4 * j;
+            double phase, mag;
+            term(g, atom, amp ? amp[j] : 1.0, detx, dety, detz, &f, &phase, &mag, NULL);
+            double px = mag * cos(phase);
+            double py = mag * sin(phase);
+            /* Kahan, cpu.py:97-105 */
+            double yx = px - cx, yy = py - cy;
+            double tx = accx + yx, ty = accy + yy;
+            cx = (tx - accx) - yx;
+            cy = (ty - accy) - yy;
+            accx = tx; accy = ty;
+        }
+        out[2 * pix] = accx;
+        out[2 * pix + 1] = accy;
+    }
+}
+
+/*
+ * Analytic FP64 gradient of a loss L(I), I_p = |E_p|^2, given dL/dI_p.
+ * With t_pj = a_j m_pj e^{i theta_pj} and w_p = (dL/dI_p) conj(E_p):
+ *   dL/dphi_j = -2 sum_p Im(w_p t_pj)
+ *   dL/dr_j   = sum_p [ -2 Im(w_p t_pj) grad_r theta_pj + 2 Re(w_p t_pj) grad_r log m_pj ]
+ *   dL/da_j   =  2 sum_p Re(w_p t_pj) / a_j
+ * nf:  grad_r theta = k (z_hat - (r_p - r_j)/D),  grad_r log(1/D) = (r_p - r_j)/D^2
+ * ff:  grad_r theta = -q_p (+ k/|r_p| (x_j, y_j, 0) with secondorder), m independent of r_j
+ *   field: n0 x n1 x 2 (re, im) from oracle_field_rows over all rows
+ *   dLdI:  n0 x n1
+ *   grad_pos: n x 3, grad_phase: n, grad_amp: n or NULL
+ */
+void oracle_adjoint(const oracle_geom *g, const double *atoms, const double *amp, int64_t n,
+                    const double *field, const double *dLdI,
+                    double *grad_pos, double *grad_phase, double *grad_amp, int nthreads)
+{
+    const int64_t n0 = g->n0, n1 = g->n1, npix = n0 * n1;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    double *pp = (double *)malloc(sizeof(double) * 3 * npix);
+    double *ff = (double *)malloc(sizeof(double) * 5 * npix);
+    double *ww = (double *)malloc(sizeof(double) * 2 * npix);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npix; ++p) {
+        double detx, dety, detz;
+        pixel_position(g, p / n1, p % n1, &detx, &dety, &detz);
+        pp[3 * p] = detx; pp[3 * p + 1] = dety; pp[3 * p + 2] = detz;
+        ffconst f = {0, 0, 0, 0, 0};
+        if (!g->nf) f = ff_setup(g, detx, dety, detz);
+        ff[5 * p] = f.qx; ff[5 * p + 1] = f.qy; ff[5 * p + 2] = f.qz;
+        ff[5 * p + 3] = f.rdist; ff[5 * p + 4] = f.c2;
+        /* w_p = G_p conj(E_p) */
+        ww[2 * p] = dLdI[p] * field[2 * p];
+        ww[2 * p + 1] = -dLdI[p] * field[2 * p + 1];
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t j = 0; j < n; ++j) {
+        const double *atom = atoms + 4 * j;
+        const double a = amp ? amp[j] : 1.0;
+        double gph = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, gre = 0.0;
+        for (int64_t p = 0; p < npix; ++p) {
+            const double detx = pp[3 * p], dety = pp[3 * p + 1], detz = pp[3 * p + 2];
+            ffconst f = {ff[5 * p], ff[5 * p + 1], ff[5 * p + 2], ff[5 * p + 3], ff[5 * p + 4]};
+            double phase, mag, dist;
+            term(g, atom, 1.0, detx, dety, detz, &f, &phase, &mag, &dist);
+            /* w_p * m e^{i phase} */
+            const double c = mag * cos(phase), s = mag * sin(phase);
+            const double re = ww[2 * p] * c - ww[2 * p + 1] * s;
+            const double im = ww[2 * p] * s + ww[2 * p + 1] * c;
+            gph += im;
+            gre += re;
+            double tx, ty, tz, lx = 0.0, ly = 0.0, lz = 0.0;
+            if (g->nf) {
+                const double dx = detx - atom[0], dy = dety - atom[1], dz = detz - atom[2];
+                const double rd = 1.0 / dist;
+                tx = -g->k * dx * rd;
+                ty = -g->k * dy * rd;
+                tz = g->k * (1.0 - dz * rd);
+                if (g->scale) { lx = dx * rd * rd; ly = dy * rd * rd; lz = dz * rd * rd; }
+            } else {
+                tx = -f.qx; ty = -f.qy; tz = -f.qz;
+                if (g->secondorder) { tx += 2.0 * f.c2 * atom[0]; ty += 2.0 * f.c2 * atom[1]; }
+            }
+            gx += -im * tx + re * lx;
+            gy += -im * ty + re * ly;
+            gz += -im * tz + re * lz;
+        }
+        grad_phase[j] = -2.0 * a * gph;
+        grad_pos[3 * j] = 2.0 * a * gx;
+        grad_pos[3 * j + 1] = 2.0 * a * gy;
+        grad_pos[3 * j + 2] = 2.0 * a * gz;
+        if (grad_amp) grad_amp[j] = 2.0 * gre;
+    }
+    free(pp); free(ff); free(ww);
+}
+
+int oracle_abi_version(void) { return 1; }

@@ -1,0 +1,69 @@
+"""The C-ABI library without a GPU: it loads, exports every declared symbol, and rejects bad
+arguments with error codes and messages (no compute calls)."""
+import ctypes
+import os
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from idi_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        from idi_amd.build import build
+
+        build()
+    return _lib.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    from idi_amd import _lib
+
+    names = _lib.declared_functions()
+    assert len(names) == 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.idisim_version() == 1
+
+
+def test_argument_errors_without_device(lib):
+    from idi_amd import _lib
+
+    h = ctypes.c_void_p()
+    assert lib.idisim_create(ctypes.byref(h), None, 1, 0) == -1
+    assert b"NULL" in lib.idisim_last_error()
+    g = _lib.GeometryT()
+    g.n0, g.n1, g.pixelsize, g.detz, g.k = 0, 4, 1.0, 1.0, 1.0
+    assert lib.idisim_create(ctypes.byref(h), ctypes.byref(g), 1, 0) == -1
+    g.n0 = 4
+    assert lib.idisim_create(ctypes.byref(h), ctypes.byref(g), 7, 0) == -1
+    g.k = -1.0
+    assert lib.idisim_create(ctypes.byref(h), ctypes.byref(g), 1, 0) == -1
+    assert lib.idisim_forward(None, None, None, 0, 0.0, None, 0, None) == -1
+    assert lib.idisim_forward_batched(None, None, None, None, 1, 0.0, None, 0, None) == -1
+    assert lib.idisim_adjoint(None, None, None, 0, 0.0, None, None, None, None, None, None) == -1
+    assert lib.idisim_set_option(None, 1, 1) == -1
+    assert lib.idisim_get_plan(None, None) == -1
+    assert lib.idisim_extent(None, 5, None, None) == -1
+    assert lib.idisim_destroy(None) == 0
+
+
+def test_no_cpu_fallback():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("this machine has a GPU")
+    from idi_amd.simulation import hip, simobj
+
+    with pytest.raises(RuntimeError):
+        hip.simulate(1, simobj.sphere(100, 10, 1), 4, 10, 1000, verbose=False)
+
+
+def test_header_declares_reference_counterparts():
+    """Each entry point's comment names what it replaces in the reference (file:line) or says it has none."""
+    text = open(os.path.join(ROOT, "include", "idisim.h")).read()
+    for token in ("sim.cu:141-149", "cpu.py:68-107", "cuda.py:53-86", "no reference counterpart", "No reference"):
+        assert token in text, token

@@ -1,0 +1,222 @@
+"""Parity of the HIP forward kernels (through the C ABI) with the FP64 oracle.
+
+Bars: the literal FP64 kernel matches the oracle to the FP64 formula's own
+rounding (per-term phase error of a few ulps of k|r_p|); the FP32-split kernel
+matches to <= 1e-4 relative L2 on the intensity (and on the field), the
+tolerance of the metric; typically it reaches ~1e-6.
+"""
+import numpy as np
+import pytest
+
+from conftest import K_UM, fp64_phase_tolerance, golden_field, golden_geometry, golden_term_scale
+from gpu_util import atoms_ball, intensity, make_handle, rel_l2, run, sc_ball
+
+pytestmark = pytest.mark.gpu
+
+SPLIT_RTOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+
+    from idi_amd import _lib
+
+    assert torch.cuda.is_available(), "GPU tests need a ROCm GPU"
+    _lib.load()
+    return _lib
+
+
+def test_golden_fp64(lib, golden):
+    for case in golden:
+        print("golden fp64", case["name"], flush=True)
+        g = golden_geometry(case)
+        f, _ = run(lib, g, np.array(case["atoms"]), None if case["amp"] is None else np.array(case["amp"]),
+                   precision=lib.PREC_FP64)
+        tol = 2 * fp64_phase_tolerance(case) * golden_term_scale(case)
+        err = np.abs(f - golden_field(case)).max()
+        assert err <= tol, (case["name"], err, tol)
+
+
+def test_golden_split(lib, golden):
+    for case in golden:
+        print("golden split", case["name"], flush=True)
+        g = golden_geometry(case)
+        f, h = run(lib, g, np.array(case["atoms"]), None if case["amp"] is None else np.array(case["amp"]))
+        plan = h.plan()
+        ref = golden_field(case)
+        scale = golden_term_scale(case)
+        err = np.abs(f - ref).max() / scale
+        if plan["fallback"]:
+            # no tile met the bound (extreme near field): FP64 kernel, complex64 output
+            assert "reftest" in case["name"], case["name"]
+            assert err <= 2e-7 + 2 * fp64_phase_tolerance(case), (case["name"], err)
+        else:
+            # per-term phase error <= plan bound (rev) -> field error <= 2 pi bound * sum|a| (+ FP32 output)
+            assert err <= 2 * np.pi * plan["phase_bound"] + 2e-7, (case["name"], err, plan)
+            assert plan["phase_bound"] <= 3e-6
+
+
+def _c1(rng):
+    from oracle import Geometry
+
+    return Geometry(128, 128, 75.0, 1e5, K_UM, nf=True), atoms_ball(rng, 1000, 0.05)
+
+
+def test_ball_1000_128px_fp64_full_image(lib, oracle_mod):
+    rng = np.random.default_rng(1000)
+    g, at = _c1(rng)
+    ref = oracle_mod.field_c(g, at)
+    f, _ = run(lib, g, at, precision=lib.PREC_FP64)
+    assert rel_l2(f, ref) <= 1e-7
+    assert rel_l2(intensity(f), intensity(ref)) <= 1e-7
+
+
+def test_ball_1000_128px_split_full_image(lib, oracle_mod):
+    rng = np.random.default_rng(1000)
+    g, at = _c1(rng)
+    ref = oracle_mod.field_c(g, at)
+    f, h = run(lib, g, at)
+    assert h.plan()["fallback"] == 0
+    e_i = rel_l2(intensity(f), intensity(ref))
+    e_f = rel_l2(f, ref)
+    print("split relL2 intensity", e_i, "field", e_f, h.plan())
+    assert e_i <= SPLIT_RTOL and e_f <= SPLIT_RTOL
+
+
+def test_sc_100k_512px_rows(lib, oracle_mod):
+    """100k of the 155,331 simple-cubic sites (a = 0.3 nm) within 10 nm, 512^2 px of 75 um at 0.2 m."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(2000)
+    at, nsites = sc_ball(rng, 100_000, 3e-4, 0.01)
+    assert nsites == 155_331
+    g = Geometry(512, 512, 75.0, 2e5, K_UM, nf=True)
+    rows = [0, 131, 256, 511]
+    ref = oracle_mod.field_c(g, at, rows=rows)
+    fs, hs = run(lib, g, at)
+    f64, _ = run(lib, g, at, precision=lib.PREC_FP64)
+    e_split = rel_l2(intensity(fs[rows]), intensity(ref))
+    e_64 = rel_l2(intensity(f64[rows]), intensity(ref))
+    e_full = rel_l2(intensity(fs), intensity(f64))
+    print("sc100k: split vs oracle", e_split, "fp64 vs oracle", e_64, "split vs fp64 full", e_full, hs.plan())
+    assert e_64 <= 1e-7
+    assert e_split <= SPLIT_RTOL
+    assert e_full <= SPLIT_RTOL
+    assert rel_l2(fs, f64) <= SPLIT_RTOL
+
+
+MODES = [
+    dict(nf=True),
+    dict(nf=False),
+    dict(nf=True, scale=True),
+    dict(nf=False, scale=True),
+    dict(nf=False, secondorder=True),
+    dict(nf=False, secondorder=True, scale=True),
+    dict(nf=True, detangles=(0.3, -0.2), detoffset=(1500.0, -700.0)),
+    dict(nf=False, detangles=(-0.1, 0.25), detoffset=(-300.0, 900.0)),
+]
+
+
+@pytest.mark.parametrize("mode", range(len(MODES)))
+def test_modes_ragged(lib, oracle_mod, mode):
+    """Every phase model on ragged sizes: odd detector, N not a multiple of the 256-emitter chunk, amplitudes."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(300 + mode)
+    m = dict(MODES[mode])
+    g = Geometry(37, 53, 150.0, 1e5, K_UM, detangles=m.pop("detangles", (0.0, 0.0)),
+                 detoffset=m.pop("detoffset", (0.0, 0.0)), **m)
+    for n, use_amp in ((1, False), (257, True), (700, False)):
+        at = atoms_ball(rng, n, 0.04)
+        amp = rng.uniform(0.5, 1.0, n) if use_amp else None
+        ref = oracle_mod.field_c(g, at, amp)
+        f64, _ = run(lib, g, at, amp, precision=lib.PREC_FP64)
+        fs, h = run(lib, g, at, amp)
+        assert rel_l2(f64, ref) <= 1e-7, (mode, n)
+        assert rel_l2(fs, ref) <= SPLIT_RTOL, (mode, n, rel_l2(fs, ref), h.plan())
+        Is, _ = run(lib, g, at, amp, kind=lib.OUT_INTENSITY)
+        assert rel_l2(Is, intensity(ref)) <= SPLIT_RTOL
+        I64, _ = run(lib, g, at, amp, precision=lib.PREC_FP64, kind=lib.OUT_INTENSITY)
+        assert rel_l2(I64, intensity(ref)) <= 1e-7
+
+
+def test_tile_split_summode_variants_agree(lib, oracle_mod):
+    from oracle import Geometry
+
+    rng = np.random.default_rng(44)
+    g = Geometry(70, 90, 100.0, 1e5, K_UM, nf=True)
+    at = atoms_ball(rng, 3000, 0.03)
+    ref = oracle_mod.field_c(g, at)
+    for tile in range(4):
+        for split in (1, 3, 8):
+            for summode in (0, 1):
+                f, h = run(lib, g, at, tile=tile, split=split, summode=summode)
+                e = rel_l2(f, ref)
+                assert e <= SPLIT_RTOL, (tile, split, summode, e)
+                p = h.plan()
+                assert (p["tile"], p["split"], p["summode"]) == (tile, split, summode)
+    for tile in range(4):
+        for split in (1, 5):
+            f, _ = run(lib, g, at, precision=lib.PREC_FP64, tile=tile, split=split)
+            assert rel_l2(f, ref) <= 1e-7, (tile, split)
+
+
+def test_empty_and_errors(lib):
+    import torch
+
+    from oracle import Geometry
+
+    g = Geometry(8, 8, 100.0, 1e5, K_UM, nf=True)
+    h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+    out = torch.full((8, 8), 3.0, dtype=torch.complex64, device="cuda")
+    h.forward(torch.empty((0, 4), dtype=torch.float64, device="cuda"), None, out)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(out).item() == 0
+    with pytest.raises(ValueError):
+        h.forward(torch.zeros((4, 3), dtype=torch.float64, device="cuda"))
+    with pytest.raises(lib.IdisimError):
+        h.set_option(lib.OPT_TILE, 9)
+    with pytest.raises(lib.IdisimError):
+        h.set_option(999, 1)
+    with pytest.raises(lib.IdisimError):
+        lib.Handle(0, 8, 100.0, 1e5, K_UM)
+    # adjoint is FP32-split only
+    h64 = make_handle(lib, g, lib.PREC_FP64)
+    e = torch.zeros((4, 4), dtype=torch.float64, device="cuda")
+    with pytest.raises(lib.IdisimError):
+        h64.adjoint(e, torch.zeros((8, 8), dtype=torch.complex64, device="cuda"),
+                    torch.zeros((8, 8), dtype=torch.float32, device="cuda"))
+
+
+def test_extent_measured_on_device(lib):
+    import ctypes
+
+    import torch
+
+    rng = np.random.default_rng(5)
+    at = atoms_ball(rng, 10_000, 0.05)
+    d = torch.from_numpy(at).cuda()
+    ext = ctypes.c_double()
+    lib.check(lib.load().idisim_extent(ctypes.c_void_p(d.data_ptr()), len(at), ctypes.byref(ext), None))
+    assert ext.value == pytest.approx(np.linalg.norm(at[:, :3], axis=1).max(), rel=1e-15)
+
+
+def test_fallback_and_strict_precision(lib, oracle_mod):
+    """Extreme near field (10 um sphere at 1 mm, the geometry of the reference's shape test)."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(6)
+    k = 2 * np.pi / (1.24 / 100)
+    g = Geometry(4, 4, 10.0, 1000.0, k, nf=True)
+    at = atoms_ball(rng, 100, 10.0)
+    ref = oracle_mod.field_c(g, at)
+    f, h = run(lib, g, at)
+    assert h.plan()["fallback"] == 1
+    assert rel_l2(f, ref) <= 1e-6
+    h.set_option(lib.OPT_FALLBACK, 0)
+    import torch
+
+    with pytest.raises(lib.IdisimError) as ei:
+        h.forward(torch.from_numpy(at).cuda())
+    assert ei.value.code == lib.E_PRECISION
