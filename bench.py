@@ -1,0 +1,291 @@
+"""Benchmark: emitter-pixel pair evaluations per second on one MI355X, FP32-split vs FP64.
+
+Default workload (one step = one near-field image through the C ABI, emitters
+already resident in HBM):
+  100,000 of the 155,331 simple-cubic sites (a = 0.3 nm) inside a 10 nm ball,
+  random phases, unit amplitudes; 512 x 512 pixels of 75 um at z = 0.2 m;
+  lambda = 0.154 nm.  2.62144e10 pairs per step.
+The FP32-split kernel is timed for --steps steps after --warmup steps, with HIP
+events on the stream the library launches on (torch's current stream).  The
+literal FP64 kernel is timed the same way as the comparator.  The FP32-split
+image is checked against the FP64 GPU image (full) and against the FP64 oracle
+on the rows the CPU baseline computes.
+
+Other workloads (--workload): batched (256 shots, N_s ~ U{1000..50000} in a
+30 nm ball, 256^2 px of 100 um at 0.1 m, one persistent graph-replayed launch),
+large (1e6 emitters in three balls, amplitudes U[0.5, 1], 1024^2 px of 50 um at
+0.5 m, 16 shots), adjoint (4096 emitters, 256^2 px of 75 um at 0.1 m).
+
+Prints ONE JSON line (rank 0).  --gpus must be 1: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LAMBDA_UM = 1.54e-4
+K = 2 * np.pi / LAMBDA_UM
+METRIC = "emitter-pixel pair evals/s, 1 GPU: FP32-split vs FP64 at <=1e-4 rel err"
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (published, packed)
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector (published)
+OPS_PER_PAIR_NF = 23       # near field, unit amplitude: 3 sub, 3 mul + 2 add, sqrt, 4 (phase), cos, sin, 8 (Kahan)
+OPS_PER_PAIR_NF_AMP = 25
+OPS_PER_PAIR_ADJ = 33
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sc_ball(rng, n, a, r):
+    m = int(np.ceil(r / a))
+    g = np.arange(-m, m + 1) * a
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pts = pts[np.linalg.norm(pts, axis=1) <= r * (1 + 1e-12)]
+    idx = rng.choice(len(pts), n, replace=False)
+    return np.concatenate([pts[idx], rng.uniform(0, 2 * np.pi, (n, 1))], 1), len(pts)
+
+
+def ball(rng, n, r):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v * r * rng.uniform(0, 1, (n, 1)) ** (1 / 3)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def host_threads():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+class StepTimer:
+    """HIP events around each step on torch's current stream (the library's launch stream)."""
+
+    def __init__(self):
+        import torch
+
+        self.torch = torch
+        self.ev = []
+
+    def run(self, fn, steps):
+        torch = self.torch
+        pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, b in pairs:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = [a.elapsed_time(b) for a, b in pairs]
+        return wall, ms
+
+
+def cpu_baseline_rows(oracle, geom, atoms, amp, budget_s, nthreads):
+    """Time the FP64 oracle on evenly spaced rows until ~budget_s of CPU work; returns rows, field, rate."""
+    n0 = geom.n0
+    t0 = time.perf_counter()
+    first = [n0 // 2]
+    f0 = oracle.field_c(geom, atoms, amp, rows=first, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    per_row = max(dt, 1e-3)
+    nrows = int(min(n0, max(4, budget_s / per_row)))
+    rows = np.unique(np.linspace(0, n0 - 1, nrows).round().astype(np.int64))
+    t0 = time.perf_counter()
+    f = oracle.field_c(geom, atoms, amp, rows=rows, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    del f0
+    return rows, f, len(rows) * geom.n1 * len(atoms) / dt, dt
+
+
+def bench_single(args):
+    import torch
+
+    import oracle
+    from idi_amd import _lib
+
+    rng = np.random.default_rng(2000)
+    atoms, nsites = sc_ball(rng, 100_000, 3e-4, 0.01)
+    n0 = n1 = 512
+    geom = oracle.Geometry(n0, n1, 75.0, 2e5, K, nf=True)
+    pairs = len(atoms) * n0 * n1
+    d = torch.from_numpy(atoms).cuda()
+    ext = float(np.linalg.norm(atoms[:, :3], axis=1).max())
+
+    hs = _lib.Handle(n0, n1, 75.0, 2e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    out = hs.forward(d, None, None, _lib.OUT_FIELD, ext)  # autotune + first run
+    torch.cuda.synchronize()
+    plan = hs.plan()
+    log("fp32-split plan", plan)
+    timer = StepTimer()
+    timer.run(lambda: hs.forward(d, None, out, _lib.OUT_FIELD, ext), args.warmup)
+    wall, ms = timer.run(lambda: hs.forward(d, None, out, _lib.OUT_FIELD, ext), args.steps)
+    split_ms = float(np.mean(ms))
+    value = pairs * args.steps / wall
+    field_split = out.cpu().numpy()
+    log(f"fp32-split: {split_ms:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} ms/step")
+
+    h64 = _lib.Handle(n0, n1, 75.0, 2e5, K, nf=True, precision=_lib.PREC_FP64)
+    out64 = h64.forward(d, None, None, _lib.OUT_FIELD, ext)
+    torch.cuda.synchronize()
+    log("fp64 plan", h64.plan())
+    s64 = max(1, min(args.steps, 5))
+    timer.run(lambda: h64.forward(d, None, out64, _lib.OUT_FIELD, ext), 1)
+    wall64, ms64 = timer.run(lambda: h64.forward(d, None, out64, _lib.OUT_FIELD, ext), s64)
+    value64 = pairs * s64 / wall64
+    field64 = out64.cpu().numpy()
+    log(f"fp64: {float(np.mean(ms64)):.3f} ms/step")
+
+    I_s, I_64 = np.abs(field_split) ** 2, np.abs(field64) ** 2
+    rel_vs_fp64 = rel_l2(I_s, I_64)
+
+    cpu = None
+    rel_vs_oracle = None
+    rel64_vs_oracle = None
+    if not args.no_cpu:
+        nthreads = host_threads()
+        rows, fref, rate, dt = cpu_baseline_rows(oracle, geom, atoms, None, args.cpu_budget, nthreads)
+        Iref = np.abs(fref) ** 2
+        rel_vs_oracle = rel_l2(I_s[rows], Iref)
+        rel64_vs_oracle = rel_l2(I_64[rows], Iref)
+        cpu = {"value": rate, "unit": "pair-evals/s", "cores": nthreads, "kind": "port",
+               "sample": f"{len(rows)} evenly spaced rows x {n1} px x {len(atoms)} emitters "
+                         f"({len(rows) * n1 * len(atoms):.3e} pairs, {dt:.1f} s) of the same image; "
+                         f"FP64 C/OpenMP restatement of cpu.py:30-107, {cpu_model()}"}
+    achieved = pairs * OPS_PER_PAIR_NF / (split_ms * 1e-3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "near-field image: 100000 of 155331 simple-cubic sites (a=0.3 nm) in a 10 nm ball, "
+                               "random phases, 512x512 px of 75 um at z=0.2 m, lambda=0.154 nm",
+                   "emitters": len(atoms), "pixels": n0 * n1, "pairs_per_step": pairs,
+                   "precision": "FP32-split (FP64 range-reduced phase, FP32 sincos + chunked Kahan sums)",
+                   "plan": plan},
+        "fp64": {"value": value64, "ms_per_step": wall64 * 1e3 / s64, "kernel_ms": float(np.mean(ms64)),
+                 "plan": h64.plan()},
+        "speedup_vs_fp64": value / value64,
+        "rel_l2_intensity": {"fp32split_vs_fp64_gpu_full": rel_vs_fp64,
+                             "fp32split_vs_oracle_rows": rel_vs_oracle,
+                             "fp64_gpu_vs_oracle_rows": rel64_vs_oracle},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": args.traffic,
+                     "ops_per_pair": OPS_PER_PAIR_NF, "kernel_ms": split_ms,
+                     "fp64_frac": pairs * OPS_PER_PAIR_NF / (float(np.mean(ms64)) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+    }
+    return result
+
+
+def bench_batched(args):
+    import torch
+
+    from idi_amd import _lib
+
+    rng = np.random.default_rng(3000)
+    nshots = 256
+    sizes = rng.integers(1000, 50001, nshots)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    pos = ball(rng, int(offsets[-1]), 0.03)
+    n0 = n1 = 256
+    pairs = int(offsets[-1]) * n0 * n1
+    ext = 0.03
+    d = torch.empty((int(offsets[-1]), 4), dtype=torch.float64, device="cuda")
+    d[:, :3] = torch.from_numpy(pos).cuda()
+    h = _lib.Handle(n0, n1, 100.0, 1e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    out = torch.empty((nshots, n0, n1), dtype=torch.float32, device="cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1)
+
+    def step():
+        # fresh phases for every shot, then one persistent graph-replayed launch
+        d[:, 3] = torch.rand(d.shape[0], generator=gen, device="cuda", dtype=torch.float64) * (2 * np.pi)
+        h.forward_batched(d, offsets, None, out, _lib.OUT_INTENSITY, ext)
+
+    step()
+    torch.cuda.synchronize()
+    timer = StepTimer()
+    timer.run(step, args.warmup)
+    wall, ms = timer.run(step, args.steps)
+    return {"metric": METRIC + " [batched 256 shots]", "value": pairs * args.steps / wall, "unit": "pair-evals/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "256 shots, N_s~U{1000..50000} uniform in a 30 nm ball, fresh phases, "
+                                   "256x256 px of 100 um at z=0.1 m, near field", "pairs_per_step": pairs,
+                       "plan": h.plan()},
+            "roofline": {"bound": "valu", "achieved": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12,
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                         "traffic": None}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="single", choices=["single", "batched"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the bench, run one elementwise copy of MIB MiB (reads and writes MIB MiB) "
+                         "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design only)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    if args.workload == "single":
+        res = bench_single(args)
+    else:
+        res = bench_batched(args)
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.rand(n, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)  # one elementwise kernel: n*4 bytes read, n*4 bytes written
+        torch.cuda.synchronize()
+        res["copy_probe_bytes"] = n * 4
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

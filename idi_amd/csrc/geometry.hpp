@@ -23,6 +23,8 @@
 // remainder (plan_bound in idisim.hip).  Far field (with or without
 // secondorder) is exact up to rounding.  Only A, W, r', dq, lam are rounded to
 // FP32; |dq . r'| is a few revolutions, so the FP32 phase keeps ~1e-7 rev.
+// FP contraction is off in these FP64 helpers, so every kernel that inlines
+// them forms bit-identical records (single-shot and batched results agree).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -39,6 +41,7 @@ struct Geom {
     double M[9];
     int n0, n1;
     int rot, nf, scale, secondorder;
+    int foldw;   // set per launch: add W_cj into A_cj (lam_p ~ 1 within the bound)
 };
 
 struct Frame {
@@ -51,6 +54,7 @@ struct Frame {
 // cpu.py:76-80 at a (possibly fractional) pixel index
 __host__ __device__ inline void pixel_pos(const Geom& g, double fx, double fy, double r[3])
 {
+#pragma clang fp contract(off)
     const double dx = (fx - g.half0) * g.pix + g.off0;
     const double dy = (fy - g.half1) * g.pix + g.off1;
     const double dz = g.detz;
@@ -67,6 +71,7 @@ __host__ __device__ inline void pixel_pos(const Geom& g, double fx, double fy, d
 
 __host__ __device__ inline Frame make_frame(const double r[3])
 {
+#pragma clang fp contract(off)
     Frame f;
     const double rxy2 = r[0] * r[0] + r[1] * r[1];
     f.R = sqrt(rxy2 + r[2] * r[2]);
@@ -103,6 +108,7 @@ struct SplitRec {
 __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c, double ex,
                                                    double ey, double ez, double ephi, double ea)
 {
+#pragma clang fp contract(off)
     const double ucr = c.u[0] * ex + c.u[1] * ey + c.u[2] * ez;
     const double tau = -g.krev * ucr + ephi * kInv2Pi;
     double W = 0.0, sig = 0.0, am = ea, s = 0.0;
@@ -121,7 +127,8 @@ __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c
     } else if (g.secondorder) {
         W = g.krev * (ex * ex + ey * ey) / (2.0 * c.R);
     }
-    const double A = tau - rint(tau);
+    const double tw = g.foldw ? tau + W : tau;
+    const double A = tw - rint(tw);
     const double f1 = 1.0 + sig;
     SplitRec r;
     r.x = (float)(ex * f1);

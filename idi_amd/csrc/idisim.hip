@@ -43,9 +43,10 @@ int fail(int code, const std::string& msg)
         if (e_ != hipSuccess) return fail(IDISIM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Largest absolute error of v_sin_f32 / v_cos_f32 on [-0.5, 0.5] rev, measured
-// on MI355X against FP64 sin/cos (in revolutions of equivalent phase).
-constexpr double kSinCosErrRev = 2.0e-7;
+// Largest error of v_sin_f32 / v_cos_f32 as an equivalent phase: measured on
+// MI355X against FP64 sin/cos over 4M arguments in [-64, 64] rev, max |dz| =
+// 1.29e-7 (2.05e-8 rev); with margin.
+constexpr double kSinCosErrRev = 3.0e-8;
 
 struct TuneKey {
     int kind, has_amp, batched, nbucket;
@@ -71,11 +72,12 @@ struct idisim_handle {
     Geom g{};
     // options
     int autotune = 1, force_tile = -1, force_split = 0, summode = 0, use_graph = 1, fallback = 1;
-    double tol_rev = 3e-6;
+    double tol_rev = 5e-6;
     // planner tables
     double du_max[kNumTiles] = {0, 0, 0, 0};
     double dR_max[kNumTiles] = {0, 0, 0, 0};
     double Rmin = 0.0, Rmax = 0.0;
+    double rho = 0.0;   // emitter extent of the current call
     // device scratch
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -146,6 +148,19 @@ void planner_tables(idisim_handle* h)
     }
 }
 
+// Error (revolutions) of folding W_cj into A_cj, i.e. of replacing lam_p W_cj
+// by W_cj: W_max max|lam_p - 1|.
+double fold_error(const idisim_handle* h, int v, double rho)
+{
+    const Geom& g = h->g;
+    if (!g.nf && !g.secondorder) return 0.0;
+    const double dR = h->dR_max[v];
+    const double Rlo = h->Rmin - dR - rho;
+    if (!(Rlo > 0.0)) return std::numeric_limits<double>::infinity();
+    const double W = g.krev * rho * rho / (2.0 * Rlo);
+    return W * dR / (h->Rmin - dR);
+}
+
 // A-priori bound of the per-term phase error (revolutions) of tile variant v
 // for emitters within |r_j| <= rho.
 double plan_bound(const idisim_handle* h, int v, double rho)
@@ -169,7 +184,7 @@ double plan_bound(const idisim_handle* h, int v, double rho)
     const double ulp = std::ldexp(1.0, std::ilogb(tau) - 23);
     const double u24 = std::ldexp(1.0, -24);
     double e = 2.0 * ulp;                // four FMA roundings of tau, each <= ulp/2
-    e += 6.0 * u24 * B;                  // dq and r' rounded to FP32
+    e += 2.0 * u24 * B;                  // dq and r' rounded to FP32: sum_i |dq_i r_i| <= B
     e += 0.5 * u24 + 2.0 * u24 * W;      // A, W and lam rounded to FP32
     if (g.nf) {
         const double R3 = Rlo * Rlo * Rlo;
@@ -182,6 +197,8 @@ double plan_bound(const idisim_handle* h, int v, double rho)
         if (g.scale) e += (ds / Rlo + rho * dR / (Rlo * Rlo)) * kInv2Pi;  // amplitude, as phase
     }
     e += kSinCosErrRev;
+    const double fe = fold_error(h, v, rho);
+    if (fe <= 0.1 * h->tol_rev) e += fe;  // W folded into A
     return e;
 }
 
@@ -279,7 +296,12 @@ int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
     const Geom& g = h->g;
     const size_t npix = (size_t)g.n0 * g.n1;
     if (c.split_path) {
-        const bool w = g.nf || g.secondorder;
+        Geom gl = g;
+        bool w = g.nf || g.secondorder;
+        if (w && fold_error(h, tile, h->rho) <= 0.1 * h->tol_rev) {
+            gl.foldw = 1;
+            w = false;
+        }
         const bool a = (c.amp != nullptr) || (g.nf && g.scale);
         float4* partial = nullptr;
         if (splits > 1) {
@@ -287,9 +309,9 @@ int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
             if (rc) return rc;
             partial = static_cast<float4*>(h->scratch);
         }
-        IDISIM_HIP(launch_fwd_split(tile, w, a, h->summode, g, c.emit, c.amp, c.n, splits, partial, c.out, c.kind,
+        IDISIM_HIP(launch_fwd_split(tile, w, a, h->summode, gl, c.emit, c.amp, c.n, splits, partial, c.out, c.kind,
                                     c.st));
-        if (splits > 1) IDISIM_HIP(launch_finalize_split(g, partial, splits, c.out, c.kind, c.st));
+        if (splits > 1) IDISIM_HIP(launch_finalize_split(gl, partial, splits, c.out, c.kind, c.st));
         return IDISIM_OK;
     }
     // FP64 kernels; a complex64/float32 output (fallback of an FP32-split
@@ -452,10 +474,15 @@ int launch_batched(idisim_handle* h, const BatchCall& c, int tile, int blocks_pe
     const Geom& g = h->g;
     const int nblocks = std::max(1, blocks_per_cu * h->num_cus);
     if (c.split_path) {
-        const bool w = g.nf || g.secondorder;
+        Geom gl = g;
+        bool w = g.nf || g.secondorder;
+        if (w && fold_error(h, tile, h->rho) <= 0.1 * h->tol_rev) {
+            gl.foldw = 1;
+            w = false;
+        }
         const bool a = (c.amp != nullptr) || (g.nf && g.scale);
-        IDISIM_HIP(launch_batched_split(tile, w, a, h->summode, g, c.emit, c.amp, h->d_items, h->nitems, h->d_counter,
-                                        nblocks, c.out, c.kind, st));
+        IDISIM_HIP(launch_batched_split(tile, w, a, h->summode, gl, c.emit, c.amp, h->d_items, h->nitems,
+                                        h->d_counter, nblocks, c.out, c.kind, st));
     } else {
         IDISIM_HIP(launch_batched_fp64(tile, g, c.emit, c.amp, h->d_items, h->nitems, h->d_counter, nblocks, c.out,
                                        c.kind, st));
@@ -466,9 +493,10 @@ int launch_batched(idisim_handle* h, const BatchCall& c, int tile, int blocks_pe
 int occupancy(idisim_handle* h, const BatchCall& c, int tile)
 {
     const Geom& g = h->g;
-    if (c.split_path)
-        return occupancy_batched_split(tile, g.nf || g.secondorder, (c.amp != nullptr) || (g.nf && g.scale),
-                                       h->summode);
+    if (c.split_path) {
+        const bool w = (g.nf || g.secondorder) && fold_error(h, tile, h->rho) > 0.1 * h->tol_rev;
+        return occupancy_batched_split(tile, w, (c.amp != nullptr) || (g.nf && g.scale), h->summode);
+    }
     return occupancy_batched_fp64(tile);
 }
 
@@ -646,6 +674,7 @@ int idisim_extent(const double* emitters, int64_t n, double* extent_out, void* s
 // Decide FP32-split vs FP64 for a call; sets the plan's precision fields.
 static int choose_path(idisim_handle* h, double rho, std::vector<int>* tiles, bool* split_path)
 {
+    h->rho = rho;
     h->plan.fallback = 0;
     h->plan.summode = h->summode;
     h->plan.graph = 0;
@@ -741,9 +770,30 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     bool split_path = true;
     int rc = choose_path(h, rho, &tiles, &split_path);
     if (rc) return rc;
-    if (h->precision == IDISIM_PREC_FP32SPLIT && !split_path)
-        return fail(IDISIM_E_PRECISION,
-                    "idisim_forward_batched: FP32-split error bound not met; use an FP64 handle for this geometry");
+    if (h->precision == IDISIM_PREC_FP32SPLIT && !split_path) {
+        // No tile meets the bound: the FP64 kernel shot by shot, writing the
+        // complex64 / float32 output of this handle (no graph).
+        const size_t shot_bytes =
+            (size_t)h->g.n0 * h->g.n1 * out_elem_bytes(IDISIM_PREC_FP32SPLIT, out_kind);
+        for (int64_t s = 0; s < nshots; ++s) {
+            const int64_t ns = offsets[s + 1] - offsets[s];
+            void* o = static_cast<char*>(out) + (size_t)s * shot_bytes;
+            if (ns == 0) {
+                rc = fill_zero(o, shot_bytes, st);
+            } else {
+                FwdCall fc{emitters + 4 * offsets[s], amplitudes ? amplitudes + offsets[s] : nullptr, ns, o,
+                           out_kind, st, false, true};
+                rc = run_forward(h, fc, 1, heuristic_split(h, ns, 1));
+            }
+            if (rc) return rc;
+        }
+        h->plan.tile = 1;
+        h->plan.tile_rows = tile_rows(1);
+        h->plan.tile_cols = tile_cols(1);
+        h->plan.split = 1;
+        h->plan.tuned_ms = 0.0;
+        return IDISIM_OK;
+    }
     BatchCall c{emitters, amplitudes, out, out_kind, st, split_path};
 
     const TuneKey key{out_kind, amplitudes != nullptr, 1, nbucket(std::max<int64_t>(1, total / nshots))};

@@ -67,7 +67,7 @@ extern "C" {
                                      Kahan; 1: Kahan on every term (cpu.py:97-105) */
 #define IDISIM_OPT_GRAPH        5 /* 1 (default): batched calls replay a hipGraph   */
 #define IDISIM_OPT_TOL_PPB      6 /* per-term phase error bound, parts per 1e9 of a
-                                     revolution (default 3000 = 3e-6 rev)          */
+                                     revolution (default 5000 = 5e-6 rev)          */
 #define IDISIM_OPT_FALLBACK     7 /* 1 (default): FP32-split handles run the FP64
                                      kernel when no tile meets the bound; 0: error */
 

@@ -46,6 +46,16 @@ def run(lib, geom, atoms, amp=None, precision=None, kind=None, handle=None, **op
     return out.cpu().numpy(), h
 
 
+def fp64_tol(geom):
+    """relL2 bound between two FP64 evaluations of the literal near-field formula: a few ulps
+    of k |r_p| per term (D ~ |r_p| is rounded before the subtraction of z_p)."""
+    if not geom.nf:
+        return 1e-9
+    half = np.hypot(geom.n0 * geom.pixelsize, geom.n1 * geom.pixelsize) / 2
+    R = np.hypot(geom.detz, half + np.hypot(*geom.detoffset))
+    return 4 * 2.0**-52 * R * geom.k
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
 

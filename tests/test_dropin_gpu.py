@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import K_UM
-from gpu_util import rel_l2
+from gpu_util import fp64_tol, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -36,7 +36,7 @@ def test_simulate_matches_oracle(oracle_mod, settings):
     obj.rng = np.random.default_rng(7)
     g = oracle.Geometry(n0, n1, 75.0, 1e5, K_UM, detoffset=(100.0, -50.0), nf="nf" in settings,
                         scale="scale" in settings, secondorder="secondorder" in settings)
-    tol = 1e-4 if "single" in settings else 1e-7
+    tol = 1e-4 if "single" in settings else fp64_tol(g)
     for i in range(3):
         ref = oracle_mod.field_c(g, obj.get())
         assert rel_l2(res[i], ref) <= tol, (settings, i)

@@ -1,0 +1,113 @@
+"""Hand-written adjoint kernel vs the FP64 analytic gradient of the oracle and FP64 central differences.
+
+Loss L = sum_p (I_p - T_p)^2 with T from a reference configuration; gradients with
+respect to positions (3N), phases (N) and amplitudes (N).
+"""
+import numpy as np
+import pytest
+
+from conftest import K_UM
+from gpu_util import ball, make_handle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 2e-3
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+
+    from idi_amd import _lib
+
+    assert torch.cuda.is_available()
+    _lib.load()
+    return _lib
+
+
+def _problem(rng, n, r, geom, oracle_mod, sigma=5e-5):
+    at0 = np.concatenate([ball(rng, n, r), rng.uniform(0, 2 * np.pi, (n, 1))], 1)
+    target = np.abs(oracle_mod.field_c(geom, at0)) ** 2
+    at = at0.copy()
+    at[:, :3] += rng.normal(0, sigma, (n, 3))
+    return at, target
+
+
+def _gpu_grad(lib, geom, at, target, amp=None):
+    import torch
+
+    h = make_handle(lib, geom, lib.PREC_FP32SPLIT)
+    d = torch.from_numpy(at).cuda()
+    da = None if amp is None else torch.from_numpy(amp).cuda()
+    field = h.forward(d, da)
+    I = (field.real ** 2 + field.imag ** 2).double()
+    dLdI = (2 * (I - torch.from_numpy(target).cuda())).float().contiguous()
+    gpos, gph, gamp = h.adjoint(d, field, dLdI, da)
+    torch.cuda.synchronize()
+    return gpos.cpu().numpy().astype(np.float64), gph.cpu().numpy().astype(np.float64), gamp.cpu().numpy(), h
+
+
+@pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder"])
+def test_adjoint_vs_fp64_analytic(lib, oracle_mod, mode):
+    from oracle import Geometry
+
+    rng = np.random.default_rng(91)
+    g = Geometry(64, 48, 300.0, 1e5, K_UM, nf=mode == "nf", secondorder="secondorder" in mode)
+    at, target = _problem(rng, 600, 0.02, g, oracle_mod)
+    amp = rng.uniform(0.5, 1, 600)
+    field = oracle_mod.field_c(g, at, amp)
+    dLdI = 2 * (np.abs(field) ** 2 - target)
+    rpos, rph, ramp = oracle_mod.adjoint_c(g, at, dLdI, field=field, amp=amp)
+    gpos, gph, gamp, h = _gpu_grad(lib, g, at, target, amp)
+    e = (rel_l2(gpos, rpos), rel_l2(gph, rph), rel_l2(gamp, ramp))
+    print(mode, "adjoint relL2 pos/phase/amp", e, h.plan())
+    assert max(e) <= GRAD_RTOL
+
+
+def test_adjoint_4096_emitters_256px(lib, oracle_mod):
+    """4096 emitters in a 20 nm ball perturbed by N(0, 0.05 nm), 256^2 px of 75 um at 0.1 m (near field)."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(4000)
+    g = Geometry(256, 256, 75.0, 1e5, K_UM, nf=True)
+    at, target = _problem(rng, 4096, 0.02, g, oracle_mod)
+    field = oracle_mod.field_c(g, at)
+    dLdI = 2 * (np.abs(field) ** 2 - target)
+    rpos, rph, _ = oracle_mod.adjoint_c(g, at, dLdI, field=field)
+    gpos, gph, _, h = _gpu_grad(lib, g, at, target)
+    e = (rel_l2(gpos, rpos), rel_l2(gph, rph))
+    print("4096 adjoint relL2 pos/phase", e, h.plan())
+    assert max(e) <= GRAD_RTOL
+    # central differences (h = 1e-4 nm) of the FP64 oracle loss on a few coordinates
+    sel = [(5, 0), (1000, 1), (2500, 2), (4095, 0)]
+    for j, c in sel:
+        p, m = at.copy(), at.copy()
+        p[j, c] += 1e-7
+        m[j, c] -= 1e-7
+        Lp = np.sum((np.abs(oracle_mod.field_c(g, p)) ** 2 - target) ** 2)
+        Lm = np.sum((np.abs(oracle_mod.field_c(g, m)) ** 2 - target) ** 2)
+        fd = (Lp - Lm) / 2e-7
+        assert abs(fd - gpos[j, c]) <= 2e-2 * np.abs(gpos).max(), (j, c, fd, gpos[j, c], rpos[j, c])
+
+
+def test_autograd_function(lib, oracle_mod):
+    import torch
+
+    from idi_amd.simulation.autograd import intensity
+    from oracle import Geometry
+
+    rng = np.random.default_rng(92)
+    g = Geometry(32, 32, 150.0, 1e5, K_UM, nf=True)
+    at, target = _problem(rng, 300, 0.02, g, oracle_mod)
+    pos = torch.from_numpy(at[:, :3].copy()).cuda().requires_grad_(True)
+    ph = torch.from_numpy(at[:, 3].copy()).cuda().requires_grad_(True)
+    amp = torch.ones(300, dtype=torch.float64, device="cuda", requires_grad=True)
+    I = intensity(pos, ph, amp, Ndet=32, pixelsize=150.0, detz=1e5, k=K_UM, settings="single nf")
+    loss = ((I.double() - torch.from_numpy(target).cuda()) ** 2).sum()
+    loss.backward()
+    field = oracle_mod.field_c(g, at)
+    rpos, rph, ramp = oracle_mod.adjoint_c(g, at, 2 * (np.abs(field) ** 2 - target), field=field,
+                                           amp=np.ones(300))
+    assert rel_l2(pos.grad.cpu().numpy(), rpos) <= GRAD_RTOL
+    assert rel_l2(ph.grad.cpu().numpy(), rph) <= GRAD_RTOL
+    assert rel_l2(amp.grad.cpu().numpy(), ramp) <= GRAD_RTOL
