@@ -250,12 +250,106 @@ def bench_batched(args):
                          "traffic": None}}
 
 
+def three_balls(rng, n_total=1_000_000):
+    """Emitters uniform in three non-overlapping balls of radius 15/20/25 nm (5 nm gaps),
+    counts proportional to volume: 125000 / 296296 / 578704."""
+    radii = np.array([0.015, 0.020, 0.025])
+    centres = np.array([[-0.040, 0, 0], [0.0, 0, 0], [0.050, 0, 0]])
+    counts = np.rint(n_total * radii**3 / np.sum(radii**3)).astype(int)
+    counts[-1] = n_total - counts[:-1].sum()
+    return np.concatenate([c + ball(rng, k, r) for c, k, r in zip(centres, counts, radii)]), counts
+
+
+def bench_large(args):
+    import torch
+
+    from idi_amd import _lib
+
+    rng = np.random.default_rng(5000)
+    nshots = 16
+    pos, counts = three_balls(rng)
+    n = len(pos)
+    n0 = n1 = 1024
+    pairs = nshots * n * n0 * n1
+    offsets = (np.arange(nshots + 1) * n).astype(np.int64)
+    d = torch.empty((nshots * n, 4), dtype=torch.float64, device="cuda")
+    dpos = torch.from_numpy(pos).cuda()
+    for s in range(nshots):
+        d[s * n:(s + 1) * n, :3] = dpos
+    amp = torch.from_numpy(rng.uniform(0.5, 1.0, nshots * n)).cuda()
+    ext = float(np.linalg.norm(pos, axis=1).max())
+    h = _lib.Handle(n0, n1, 50.0, 5e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    out = torch.empty((nshots, n0, n1), dtype=torch.float32, device="cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+
+    def step():
+        d[:, 3] = torch.rand(d.shape[0], generator=gen, device="cuda", dtype=torch.float64) * (2 * np.pi)
+        h.forward_batched(d, offsets, amp, out, _lib.OUT_INTENSITY, ext)
+
+    step()
+    torch.cuda.synchronize()
+    timer = StepTimer()
+    timer.run(step, args.warmup)
+    wall, ms = timer.run(step, args.steps)
+    ach = pairs * OPS_PER_PAIR_NF_AMP / (np.mean(ms) * 1e-3) / 1e12
+    return {"metric": METRIC + " [1e6 emitters, 16 shots, amplitudes]", "value": pairs * args.steps / wall,
+            "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "1e6 emitters in three balls (r=15/20/25 nm, counts %s), amplitudes U[0.5,1], "
+                                   "fresh phases, 16 shots, 1024x1024 px of 50 um at z=0.5 m, near field"
+                                   % counts.tolist(), "pairs_per_step": pairs, "plan": h.plan()},
+            "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "ops_per_pair": OPS_PER_PAIR_NF_AMP}}
+
+
+def bench_adjoint(args):
+    import torch
+
+    from idi_amd import _lib
+
+    rng = np.random.default_rng(4000)
+    n = 4096
+    n0 = n1 = 256
+    pos0 = ball(rng, n, 0.02)
+    phase = rng.uniform(0, 2 * np.pi, n)
+    at0 = np.concatenate([pos0, phase[:, None]], 1)
+    at = at0.copy()
+    at[:, :3] += rng.normal(0, 5e-5, (n, 3))
+    h = _lib.Handle(n0, n1, 75.0, 1e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    target = h.forward(torch.from_numpy(at0).cuda(), None, None, _lib.OUT_INTENSITY).double()
+    d = torch.from_numpy(at).cuda()
+    ext = float(np.linalg.norm(at[:, :3], axis=1).max())
+    field = torch.empty((n0, n1), dtype=torch.complex64, device="cuda")
+    pairs = 2 * n * n0 * n1
+
+    def step():
+        h.forward(d, None, field, _lib.OUT_FIELD, ext)
+        I = (field.real * field.real + field.imag * field.imag).double()
+        dLdI = (2 * (I - target)).float()
+        h.adjoint(d, field, dLdI, None, ext, want_amp=False)
+
+    step()
+    torch.cuda.synchronize()
+    timer = StepTimer()
+    timer.run(step, args.warmup)
+    wall, ms = timer.run(step, args.steps)
+    return {"metric": METRIC + " [forward + adjoint]", "value": pairs * args.steps / wall, "unit": "pair-evals/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "4096 emitters in a 20 nm ball perturbed by N(0, 0.05 nm), loss "
+                                   "sum (I - I_target)^2, 256x256 px of 75 um at z=0.1 m, near field; "
+                                   "step = forward + adjoint (pairs counted twice)", "pairs_per_step": pairs,
+                       "plan": h.plan()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single", choices=["single", "batched"])
+    ap.add_argument("--workload", default="single", choices=["single", "batched", "large", "adjoint"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
@@ -272,10 +366,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
-    if args.workload == "single":
-        res = bench_single(args)
-    else:
-        res = bench_batched(args)
+    res = {"single": bench_single, "batched": bench_batched, "large": bench_large,
+           "adjoint": bench_adjoint}[args.workload](args)
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.rand(n, device="cuda")

@@ -21,8 +21,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libidisim.so")
-SOURCES = ["kernels.hip", "adjoint.hip", "idisim.hip"]
+SOURCES = ["kernels.hip", "kernels_mfma.hip", "adjoint.hip", "idisim.hip"]
 ARCH = "gfx950"
+# MFMA results in VGPRs: the VALU reads tau directly (no v_accvgpr_read per value)
+FILE_FLAGS = {"kernels_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def hipcc_path():
@@ -54,7 +56,8 @@ def build(verbose=False, jobs=3):
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [hipcc, *compile_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+        extra = FILE_FLAGS.get(src, [])
+        cmd = [hipcc, *compile_flags(), *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -21,8 +21,9 @@
 // The nf lines are the first-order expansion of w in (s, R) about the tile
 // centre (dw/ds = (w - s)/D, dw/dR = -w/D); the host planner bounds the
 // remainder (plan_bound in idisim.hip).  Far field (with or without
-// secondorder) is exact up to rounding.  Only A, W, r', dq, lam are rounded to
-// FP32; |dq . r'| is a few revolutions, so the FP32 phase keeps ~1e-7 rev.
+// secondorder) is exact up to rounding.  A is formed in FP64 and rounded; W,
+// r', dq, lam are FP32; |dq . r'| is a few revolutions, so the FP32 phase
+// keeps ~1e-7 rev.
 // FP contraction is off in these FP64 helpers, so every kernel that inlines
 // them forms bit-identical records (single-shot and batched results agree).
 #pragma once
@@ -98,11 +99,14 @@ __host__ __device__ inline Frame pixel_frame(const Geom& g, double fx, double fy
     return make_frame(r);
 }
 
-// Per (tile centre, emitter) record of the FP32-split kernels, formed in FP64.
+// Per (tile centre, emitter) record of the FP32-split kernels.  Only the large
+// linear phase A needs FP64 (|k_rev u_c . r_j| reaches tens of revolutions and
+// is reduced mod 1); W, sigma and the amplitude factor are small corrections
+// whose relative precision only needs ~1e-6, so they are formed in FP32.
 struct SplitRec {
     float x, y, z, A;   // r'_cj and the reduced linear phase
     float W, amp;       // curvature / second-order coefficient, amplitude
-    float sigma, s;     // kept for the adjoint: sigma_cj and s_cj (nf)
+    float sigma, s;     // sigma_cj and s_cj (nf)
 };
 
 __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c, double ex,
@@ -111,34 +115,37 @@ __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c
 #pragma clang fp contract(off)
     const double ucr = c.u[0] * ex + c.u[1] * ey + c.u[2] * ez;
     const double tau = -g.krev * ucr + ephi * kInv2Pi;
-    double W = 0.0, sig = 0.0, am = ea, s = 0.0;
+    const float fx = (float)ex, fy = (float)ey, fz = (float)ez;
+    float W = 0.f, sig = 0.f, am = (float)ea, s = 0.f;
     if (g.nf) {
-        s = ucr + ez;
-        const double hx = c.rh[1] * ez - c.rh[2] * ey;
-        const double hy = c.rh[2] * ex - c.rh[0] * ez;
-        const double hz = c.rh[0] * ey - c.rh[1] * ex;
-        const double h2 = hx * hx + hy * hy + hz * hz;
-        const double a = c.R - s;
-        const double D = sqrt(a * a + h2);
-        const double w = (a >= 0.0) ? h2 / (D + a) : D - a;
-        W = g.krev * w;
+        s = (float)(ucr + ez);
+        const float rx = (float)c.rh[0], ry = (float)c.rh[1], rz = (float)c.rh[2];
+        const float hx = ry * fz - rz * fy;
+        const float hy = rz * fx - rx * fz;
+        const float hz = rx * fy - ry * fx;
+        const float h2 = hx * hx + hy * hy + hz * hz;
+        const float R = (float)c.R;
+        const float a = R - s;
+        const float D = sqrtf(a * a + h2);
+        const float w = (a >= 0.f) ? h2 / (D + a) : D - a;
+        W = (float)g.krev * w;
         sig = (s - w) / D;
-        if (g.scale) am = ea * (c.R / D);
+        if (g.scale) am = (float)ea * (R / D);
     } else if (g.secondorder) {
-        W = g.krev * (ex * ex + ey * ey) / (2.0 * c.R);
+        W = (float)g.krev * (fx * fx + fy * fy) / (2.f * (float)c.R);
     }
-    const double tw = g.foldw ? tau + W : tau;
+    const double tw = g.foldw ? tau + (double)W : tau;
     const double A = tw - rint(tw);
-    const double f1 = 1.0 + sig;
+    const float f1 = 1.f + sig;
     SplitRec r;
-    r.x = (float)(ex * f1);
-    r.y = (float)(ey * f1);
-    r.z = (float)(ez * f1);
+    r.x = fx * f1;
+    r.y = fy * f1;
+    r.z = fz * f1;
     r.A = (float)A;
-    r.W = (float)W;
-    r.amp = (float)am;
-    r.sigma = (float)sig;
-    r.s = (float)s;
+    r.W = W;
+    r.amp = am;
+    r.sigma = sig;
+    r.s = s;
     return r;
 }
 
@@ -149,10 +156,13 @@ struct WorkItem {
     int64_t e0, e1;   // emitter range [e0, e1) in the concatenated array
 };
 
-// Pixel-tile variants: lanes form a 16 x 16 grid, each lane owns PX x PY pixels.
-constexpr int kNumTiles = 4;
-__host__ __device__ constexpr int tile_px(int v) { return v >= 2 ? 2 : 1; }
-__host__ __device__ constexpr int tile_py(int v) { return v == 0 ? 1 : (v == 3 ? 4 : 2); }
+// Pixel-tile variants.  0..3: VALU kernels, lanes form a 16 x 16 grid and each
+// lane owns PX x PY pixels (16x16, 16x32, 32x32, 32x64 pixels per workgroup).
+// 4: MFMA kernel (kernels_mfma.hip), 16 x 16 pixels per workgroup.
+constexpr int kNumTiles = 5;
+constexpr int kTileMfma = 4;
+__host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileMfma ? 2 : 1; }
+__host__ __device__ constexpr int tile_py(int v) { return v == 0 || v == kTileMfma ? 1 : (v == 3 ? 4 : 2); }
 __host__ __device__ constexpr int tile_rows(int v) { return 16 * tile_px(v); }
 __host__ __device__ constexpr int tile_cols(int v) { return 16 * tile_py(v); }
 

@@ -185,7 +185,8 @@ double plan_bound(const idisim_handle* h, int v, double rho)
     const double u24 = std::ldexp(1.0, -24);
     double e = 2.0 * ulp;                // four FMA roundings of tau, each <= ulp/2
     e += 2.0 * u24 * B;                  // dq and r' rounded to FP32: sum_i |dq_i r_i| <= B
-    e += 0.5 * u24 + 2.0 * u24 * W;      // A, W and lam rounded to FP32
+    e += 0.5 * u24 + 6.0 * u24 * W;      // A rounded; W formed in FP32; lam rounded
+    e += 4.0 * u24 * B;                  // r' = r (1 + sigma) formed in FP32
     if (g.nf) {
         const double R3 = Rlo * Rlo * Rlo;
         const double wss = Rhi * Rhi / R3;
@@ -258,6 +259,7 @@ std::vector<int> eligible_tiles(const idisim_handle* h, double rho, double* best
     std::vector<int> out;
     *best_bound = std::numeric_limits<double>::infinity();
     for (int v = 0; v < kNumTiles; ++v) {
+        if (v == kTileMfma && h->summode != 0) continue;
         const double b = plan_bound(h, v, rho);
         *best_bound = std::min(*best_bound, b);
         if (b <= h->tol_rev) out.push_back(v);
@@ -309,8 +311,13 @@ int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
             if (rc) return rc;
             partial = static_cast<float4*>(h->scratch);
         }
-        IDISIM_HIP(launch_fwd_split(tile, w, a, h->summode, gl, c.emit, c.amp, c.n, splits, partial, c.out, c.kind,
-                                    c.st));
+        if (tile == kTileMfma) {
+            if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the MFMA tile has no per-term Kahan mode");
+            IDISIM_HIP(launch_fwd_mfma(gl, c.emit, c.amp, c.n, splits, partial, c.out, c.kind, c.st));
+        } else {
+            IDISIM_HIP(launch_fwd_split(tile, w, a, h->summode, gl, c.emit, c.amp, c.n, splits, partial, c.out,
+                                        c.kind, c.st));
+        }
         if (splits > 1) IDISIM_HIP(launch_finalize_split(gl, partial, splits, c.out, c.kind, c.st));
         return IDISIM_OK;
     }
@@ -401,7 +408,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     } else {
         // heuristic: 2x2 pixels per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {2, 1, 3, 0})
+        for (int v : {kTileMfma, 2, 1, 3, 0})
             if (std::find(tiles.begin(), tiles.end(), v) != tiles.end()) {
                 ch.tile = v;
                 break;
@@ -481,8 +488,14 @@ int launch_batched(idisim_handle* h, const BatchCall& c, int tile, int blocks_pe
             w = false;
         }
         const bool a = (c.amp != nullptr) || (g.nf && g.scale);
-        IDISIM_HIP(launch_batched_split(tile, w, a, h->summode, gl, c.emit, c.amp, h->d_items, h->nitems,
-                                        h->d_counter, nblocks, c.out, c.kind, st));
+        if (tile == kTileMfma) {
+            if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the MFMA tile has no per-term Kahan mode");
+            IDISIM_HIP(launch_batched_mfma(gl, c.emit, c.amp, h->d_items, h->nitems, h->d_counter, nblocks, c.out,
+                                           c.kind, st));
+        } else {
+            IDISIM_HIP(launch_batched_split(tile, w, a, h->summode, gl, c.emit, c.amp, h->d_items, h->nitems,
+                                            h->d_counter, nblocks, c.out, c.kind, st));
+        }
     } else {
         IDISIM_HIP(launch_batched_fp64(tile, g, c.emit, c.amp, h->d_items, h->nitems, h->d_counter, nblocks, c.out,
                                        c.kind, st));
@@ -495,6 +508,7 @@ int occupancy(idisim_handle* h, const BatchCall& c, int tile)
     const Geom& g = h->g;
     if (c.split_path) {
         const bool w = (g.nf || g.secondorder) && fold_error(h, tile, h->rho) > 0.1 * h->tol_rev;
+        if (tile == kTileMfma) return occupancy_batched_mfma(!w);
         return occupancy_batched_split(tile, w, (c.amp != nullptr) || (g.nf && g.scale), h->summode);
     }
     return occupancy_batched_fp64(tile);
@@ -834,7 +848,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {2, 1, 3, 0})
+        for (int v : {kTileMfma, 2, 1, 3, 0})
             if (std::find(tiles.begin(), tiles.end(), v) != tiles.end()) {
                 ch.tile = v;
                 break;
