@@ -175,6 +175,16 @@ def bench_single(args):
                          f"({len(rows) * n1 * len(atoms):.3e} pairs, {dt:.1f} s) of the same image; "
                          f"FP64 C/OpenMP restatement of cpu.py:30-107, {cpu_model()}"}
     achieved = pairs * OPS_PER_PAIR_NF / (split_ms * 1e-3) / 1e12
+    traffic = args.traffic
+    if traffic is None:
+        # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE for this plan
+        try:
+            with open(os.path.join(ROOT, "profiles", "fwd_split_sc100k_512px_traffic.json")) as f:
+                rec = json.load(f)
+            if rec["plan"]["tile"] == plan["tile"] and rec["plan"]["split"] == plan["split"]:
+                traffic = rec["hbm_bytes_per_launch"]
+        except (OSError, KeyError, ValueError):
+            traffic = None
     result = {
         "metric": METRIC,
         "value": value,
@@ -200,7 +210,7 @@ def bench_single(args):
                              "fp32split_vs_oracle_rows": rel_vs_oracle,
                              "fp64_gpu_vs_oracle_rows": rel64_vs_oracle},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": args.traffic,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "ops_per_pair": OPS_PER_PAIR_NF, "kernel_ms": split_ms,
                      "fp64_frac": pairs * OPS_PER_PAIR_NF / (float(np.mean(ms64)) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
@@ -217,20 +227,18 @@ def bench_batched(args):
     nshots = 256
     sizes = rng.integers(1000, 50001, nshots)
     offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    pos = ball(rng, int(offsets[-1]), 0.03)
     n0 = n1 = 256
     pairs = int(offsets[-1]) * n0 * n1
     ext = 0.03
     d = torch.empty((int(offsets[-1]), 4), dtype=torch.float64, device="cuda")
-    d[:, :3] = torch.from_numpy(pos).cuda()
     h = _lib.Handle(n0, n1, 100.0, 1e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
     out = torch.empty((nshots, n0, n1), dtype=torch.float32, device="cuda")
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(1)
+    seeds = iter(range(1, 1 << 30))
 
     def step():
-        # fresh phases for every shot, then one persistent graph-replayed launch
-        d[:, 3] = torch.rand(d.shape[0], generator=gen, device="cuda", dtype=torch.float64) * (2 * np.pi)
+        # fresh positions (uniform in the 30 nm ball) and phases for every shot, drawn
+        # on the device, then one persistent graph-replayed launch over all shots
+        _lib.generate_ball(d, next(seeds), 0, ext)
         h.forward_batched(d, offsets, None, out, _lib.OUT_INTENSITY, ext)
 
     step()
@@ -241,8 +249,9 @@ def bench_batched(args):
     return {"metric": METRIC + " [batched 256 shots]", "value": pairs * args.steps / wall, "unit": "pair-evals/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-            "config": {"workload": "256 shots, N_s~U{1000..50000} uniform in a 30 nm ball, fresh phases, "
-                                   "256x256 px of 100 um at z=0.1 m, near field", "pairs_per_step": pairs,
+            "config": {"workload": "256 shots, N_s~U{1000..50000} uniform in a 30 nm ball, fresh positions and "
+                                   "phases per step drawn on the GPU (Philox), 256x256 px of 100 um at z=0.1 m, "
+                                   "near field, intensity out", "pairs_per_step": pairs,
                        "plan": h.plan()},
             "roofline": {"bound": "valu", "achieved": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -280,11 +289,11 @@ def bench_large(args):
     ext = float(np.linalg.norm(pos, axis=1).max())
     h = _lib.Handle(n0, n1, 50.0, 5e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
     out = torch.empty((nshots, n0, n1), dtype=torch.float32, device="cuda")
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(5)
+    seeds = iter(range(1, 1 << 30))
 
     def step():
-        d[:, 3] = torch.rand(d.shape[0], generator=gen, device="cuda", dtype=torch.float64) * (2 * np.pi)
+        # fresh phases for every shot (positions fixed), drawn on the device
+        _lib.generate_ball(d, next(seeds), 0, positions=False)
         h.forward_batched(d, offsets, amp, out, _lib.OUT_INTENSITY, ext)
 
     step()

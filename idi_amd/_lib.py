@@ -88,8 +88,10 @@ def load():
     lib.idisim_forward_batched.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_int64), i64, dbl, vp, ip, vp]
     lib.idisim_adjoint.argtypes = [vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp, vp]
     lib.idisim_extent.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), vp]
+    lib.idisim_generate_ball.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl,
+                                         ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
-                 "idisim_forward_batched", "idisim_adjoint", "idisim_extent"):
+                 "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball"):
         getattr(lib, name).restype = ip
     if lib.idisim_version() != 1:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
@@ -124,6 +126,21 @@ def _stream():
     import torch
 
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def generate_ball(out, seed, rng_stream=0, radius=1.0, centre=(0.0, 0.0, 0.0), positions=True, phases=True):
+    """Fill a contiguous (N, 4) float64 CUDA tensor with emitters uniform in a ball (Philox4x32-10)."""
+    import torch
+
+    _require_cuda()
+    lib = load()
+    if out.dtype != torch.float64 or out.dim() != 2 or out.shape[1] != 4 or not out.is_contiguous() or not out.is_cuda:
+        raise ValueError("out must be a contiguous (N, 4) float64 CUDA tensor")
+    c = (ctypes.c_double * 3)(*[float(v) for v in centre])
+    check(lib.idisim_generate_ball(ctypes.c_uint64(int(seed) & (2**64 - 1)),
+                                   ctypes.c_uint64(int(rng_stream) & (2**64 - 1)), int(out.shape[0]),
+                                   float(radius), c, int(bool(positions)), int(bool(phases)), _ptr(out), _stream()))
+    return out
 
 
 class Handle:

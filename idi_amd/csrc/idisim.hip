@@ -685,6 +685,20 @@ int idisim_extent(const double* emitters, int64_t n, double* extent_out, void* s
     return IDISIM_OK;
 }
 
+int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, const double* centre,
+                         int positions, int phases, double* out, void* stream)
+{
+    g_err.clear();
+    if (n < 0 || (n > 0 && !out)) return fail(IDISIM_E_INVALID, "idisim_generate_ball: bad arguments");
+    if (!(radius >= 0.0) || !std::isfinite(radius))
+        return fail(IDISIM_E_INVALID, "idisim_generate_ball: radius must be finite and >= 0");
+    if (centre && !(std::isfinite(centre[0]) && std::isfinite(centre[1]) && std::isfinite(centre[2])))
+        return fail(IDISIM_E_INVALID, "idisim_generate_ball: centre must be finite");
+    IDISIM_HIP(launch_gen_ball(seed, rng_stream, n, radius, centre, positions ? 1 : 0, phases ? 1 : 0, out,
+                               static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
 // Decide FP32-split vs FP64 for a call; sets the plan's precision fields.
 static int choose_path(idisim_handle* h, double rho, std::vector<int>* tiles, bool* split_path)
 {

@@ -44,4 +44,8 @@ hipError_t launch_adjoint(bool w, const Geom& g, const double* emit, const doubl
 
 hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out, hipStream_t st);
 
+// Emitters in a ball / random phases from Philox4x32-10 (generate.hip).
+hipError_t launch_gen_ball(uint64_t seed, uint64_t stream, int64_t n, double radius, const double* centre,
+                           int positions, int phases, double* out, hipStream_t st);
+
 }  // namespace idisim

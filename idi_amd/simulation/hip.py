@@ -24,6 +24,7 @@ Extensions (keyword arguments; the reference signatures are unchanged):
     amplitudes   (N,) per-emitter amplitudes, or a callable returning them per
                  image; default: ``simobject.get_amplitudes()`` if present
     intensity    True: return |E|^2 (float) instead of the field
+    device_rng   True: draw sphere emitters / random phases on the GPU
     simulate_batched(...)  many shots with uneven emitter counts in one
                  persistent, graph-replayed launch.
 
@@ -123,14 +124,34 @@ class _Staging:
         self.ready.record()
 
 
+def _device_source(simobject):
+    """How the device can draw this object's emitters: 'ball' (positions and phases),
+    'phases' (fixed positions, random phases) or None (host draws)."""
+    from . import simobj as _so
+
+    rnd_pos = getattr(simobject, "rndPos", True)
+    rnd_phase = getattr(simobject, "rndPhase", False)
+    if not rnd_pos and rnd_phase:
+        return "phases"
+    if type(simobject) is _so.sphere and rnd_pos and rnd_phase:
+        return "ball"
+    return None
+
+
 def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", maximg=_np.inf, detangles=(0, 0),
-                 detoffset=(0, 0), returngpu=False, *args, amplitudes=None, intensity=False, **kwargs):
+                 detoffset=(0, 0), returngpu=False, *args, amplitudes=None, intensity=False, device_rng=False,
+                 **kwargs):
     """Generator of simulated wavefields (cuda.py:16-138).
 
     Two-deep pipeline: while the GPU computes image i, the host draws the
     emitters of image i+1 and copies them through pinned memory on the same
     stream; image i is copied back (or cloned on the device) after image i+1
     has been launched.
+
+    device_rng=True (extension): a ``simobj.sphere`` is drawn on the GPU
+    (idisim_generate_ball, Philox4x32-10, seed of image i drawn from
+    ``simobject.rng``), and an object with fixed positions (rndPos False) and
+    random phases gets its phases drawn there; nothing is copied per image.
     """
     import torch
 
@@ -152,17 +173,37 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
             stages = [_Staging(simobject.N, a0 is not None, dev) for _ in range(2)]
             outs = [torch.empty((n0, n1), dtype=dtype, device=dev) for _ in range(2)]
 
-            def launch(slot, atoms, amp):
-                st = stages[slot]
-                st.upload(atoms, amp)
-                h.forward(st.d, st.da if amp is not None else None, outs[slot], kind, _extent(atoms))
+            source = _device_source(simobject) if device_rng else None
+            if source == "phases":
+                fixed = simobject.get()
+                for st in stages:
+                    st.upload(fixed, a0)
+                fixed_extent = _extent(fixed)
 
-            launch(0, simobject.get(), a0)
+            def launch(slot, amp):
+                st = stages[slot]
+                if source is None:
+                    atoms = simobject.get()
+                    st.upload(atoms, amp)
+                    ext = _extent(atoms)
+                else:
+                    seed = int(simobject.rng.integers(2**63))
+                    if source == "ball":
+                        _lib.generate_ball(st.d, seed, 0, simobject.r)
+                        ext = float(simobject.r)
+                    else:
+                        _lib.generate_ball(st.d, seed, 0, positions=False)
+                        ext = fixed_extent
+                    if amp is not None:
+                        st.da.copy_(torch.from_numpy(amp))
+                h.forward(st.d, st.da if amp is not None else None, outs[slot], kind, ext)
+
+            launch(0, a0)
             slot = 0
             while count < maximg:
                 count += 1
                 if count < maximg:
-                    launch(1 - slot, simobject.get(), _amplitudes(simobject, amplitudes))
+                    launch(1 - slot, _amplitudes(simobject, amplitudes))
                 if returngpu:
                     ret = outs[slot].clone()
                 else:
@@ -181,7 +222,8 @@ def simulate(Nimg, simobject, Ndet, pixelsize, detz, k=None, settings="double", 
 
     n0, n1 = _ndet(Ndet)
     gen = simulate_gen(simobject, (n0, n1), pixelsize, detz, k, settings=settings, maximg=Nimg, detangles=detangles,
-                       detoffset=detoffset, returngpu=returngpu, amplitudes=amplitudes, intensity=intensity)
+                       detoffset=detoffset, returngpu=returngpu, amplitudes=amplitudes, intensity=intensity,
+                       device_rng=kwargs.get("device_rng", False))
     if returngpu:
         h = get_handle((n0, n1), pixelsize, detz, simobject.k if k is None else k, settings, detangles, detoffset)
         dt = h.real_dtype if intensity else h.field_dtype

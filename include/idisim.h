@@ -148,6 +148,17 @@ int idisim_adjoint(idisim_handle *h, const double *emitters, const double *ampli
 /* Largest |r_j| of n emitters (device reduction; synchronises `stream`). */
 int idisim_extent(const double *emitters, int64_t n, double *extent_out, void *stream);
 
+/* Emitters drawn on the device (no reference counterpart): it replaces the
+   host draws of simobj.sphere (simobj.py:119-130, util/random.py:39-51) and of
+   the random phases (simobj.py:56-57).  Counter-based Philox4x32-10 with key
+   `seed`; emitter i of `rng_stream` depends only on (seed, rng_stream, i).
+     out:       n x 4 float64 (x, y, z, phi), written in place
+     positions: 1 = uniform in the ball |r - centre| <= radius, 0 = keep x, y, z
+     phases:    1 = phi ~ U[0, 2 pi), 0 = keep phi
+     centre:    HOST array of 3 doubles or NULL (origin)                    */
+int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, const double *centre,
+                         int positions, int phases, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
