@@ -62,7 +62,7 @@ _lib = None
 def declared_functions(header=HEADER):
     """Names of the functions include/idisim.h declares."""
     text = open(header).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(idisim_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:IDISIM_API\s+)?(?:const\s+)?\w+\s*\*?\s*(idisim_\w+)\s*\(", text, re.M)))
 
 
 def load():

@@ -40,6 +40,7 @@ def compile_flags():
         "-O3",
         "-std=c++17",
         "-fPIC",
+        "-fvisibility=hidden",
         "-Wall",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I", os.path.join(ROOT, "include"),

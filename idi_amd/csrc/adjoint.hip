@@ -7,76 +7,118 @@
 //                 = -q_p (+ k/|r_p| (x_j, y_j, 0) with secondorder)  (far field).
 // With the tile decomposition of geometry.hpp (u_p = u_c + du_p, dq_p = -krev du_p):
 //   near field:  z_hat - n_pj = -u_c + beta_cj - du_p,
-//                beta_cj = (rh_c (w_cj - s_cj) + r_j) / D_cj   (O(|r_j|/R), taken at the centre)
+//                beta_cj = (rh_c (w_cj - s_cj) + r_j) / D_cj = r_j / D_cj - rh_c sigma_cj
+//                (O(|r_j|/R), taken at the tile centre)
 //   far field:   -u_p = -u_c - du_p
-// so per (tile, emitter) only S = sum_p w_p e^{i theta} (complex) and
-// V = sum_p Im(.) dq_p (3 floats) are needed in FP32; they are folded into
-// FP64 accumulators with the FP64 tile-centre factors after every tile.
+// so per (tile, emitter) only S = sum_p Im(w_p e^{i theta}) and V = sum_p Im(.) dq_p
+// (3 floats) are needed in FP32 (plus sum Re(.) for dL/da); they are folded into FP64
+// accumulators with the FP64 tile-centre vector u_c after every tile.
 //
-// Layout: emitter-parallel.  A workgroup owns 256*EPL emitters (EPL per lane)
-// and a range of 16 x 16 pixel tiles; each tile's per-pixel data (dq, lam, w')
-// is formed in FP64 by one lane per pixel and staged in LDS, then read by all
-// lanes as broadcasts.  Partial gradients of each pixel range go to a scratch
-// buffer [splits][n][5] and a finalize kernel sums them in FP64.
+// Kernels:
+//   k_adj_pixels  once per call: per pixel of each 16 x 16 tile, dq_p and lam_p = R_c/R_p
+//                 relative to the tile centre and w'_p = w_p e^{2 pi i Phi_p} (the per-pixel
+//                 near-field phase, FP64), laid out tile-major for LDS staging.
+//   k_adjoint     emitter-parallel: a workgroup owns 64*EPL emitters, shared by its four
+//                 waves; each wave streams its own pixel tiles (tile t0 + 4 it + wave) through
+//                 a private LDS slot, so the 4 waves cover 4 tiles per step.  The per-wave
+//                 FP64 sums are reduced through LDS and written as partials [splits][n][5].
+//   k_adjoint_finalize  sums the partials over splits in FP64.
+// The pixel loop is not unrolled: the four emitters per lane give enough independent
+// work, and the lower register count lets four workgroups share a CU.
 #include "kernels.hpp"
 
 namespace idisim {
 
-template <int EPL, bool HAS_W>
+namespace {
+constexpr int kAdjWaves = kBlock / 64;  // waves per workgroup
+constexpr int kAdjEPL = 4;              // emitters per lane
+constexpr int kAdjTile = 256;           // pixels per 16 x 16 tile
+
+__device__ inline Frame adj_tile_centre(const Geom& g, int t, int tiles_y)
+{
+    const int tx0 = (t / tiles_y) * 16, ty0 = (t % tiles_y) * 16;
+    const int xe = min(tx0 + 16, g.n0) - 1, ye = min(ty0 + 16, g.n1) - 1;
+    return pixel_frame(g, 0.5 * (tx0 + xe), 0.5 * (ty0 + ye));
+}
+}  // namespace
+
+__global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* __restrict__ field,
+                                                         const float* __restrict__ dLdI, int tiles_y,
+                                                         float4* __restrict__ pq, float2* __restrict__ pw)
+{
+    const int t = blockIdx.x, l = threadIdx.x;
+    const int x = (t / tiles_y) * 16 + (l >> 4), y = (t % tiles_y) * 16 + (l & 15);
+    float4 q = make_float4(0.f, 0.f, 0.f, 1.f);
+    float2 w = make_float2(0.f, 0.f);
+    if (x < g.n0 && y < g.n1) {
+        const Frame c = adj_tile_centre(g, t, tiles_y);
+        const Frame f = pixel_frame(g, (double)x, (double)y);
+        q = make_float4((float)(-g.krev * (f.u[0] - c.u[0])), (float)(-g.krev * (f.u[1] - c.u[1])),
+                        (float)(-g.krev * (f.u[2] - c.u[2])), (float)(c.R / f.R));
+        const size_t idx = (size_t)x * g.n1 + y;
+        const float2 E = field[idx];
+        const double G = (double)dLdI[idx];
+        double wr = G * (double)E.x, wi = -G * (double)E.y;
+        if (g.nf) {
+            double ph = g.krev * f.RmZ;
+            ph -= rint(ph);
+            double sn, cs;
+            sincos(kTwoPi * ph, &sn, &cs);
+            const double r2 = wr * cs - wi * sn;
+            wi = wr * sn + wi * cs;
+            wr = r2;
+        }
+        w = make_float2((float)wr, (float)wi);
+    }
+    pq[(size_t)t * kAdjTile + l] = q;
+    pw[(size_t)t * kAdjTile + l] = w;
+}
+
+// HAS_W: lam_p W_cj enters the phase (nf or secondorder, not folded into A).
+// LAM:   secondorder far field unfolded: also accumulate sum_p Im(.) lam_p.
+// AMP:   also accumulate sum_p Re(.) for dL/da.
+template <int EPL, bool HAS_W, bool LAM, bool AMP>
 __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __restrict__ emit,
                                                     const double* __restrict__ amp, int64_t n,
-                                                    const float2* __restrict__ field,
-                                                    const float* __restrict__ dLdI, int tiles_y, int ntiles,
-                                                    int tiles_per_split, double* __restrict__ partial)
+                                                    const float4* __restrict__ pq, const float2* __restrict__ pw,
+                                                    int tiles_y, int ntiles, int tiles_per_split,
+                                                    double* __restrict__ partial)
 {
-    __shared__ float4 sp0[kBlock];  // dqx, dqy, dqz, lam
-    __shared__ float2 sp1[kBlock];  // w'_p (per-pixel phase removed)
-    const int tid = threadIdx.x;
-    const int64_t ebase = (int64_t)blockIdx.x * (kBlock * EPL);
+    constexpr int M = 64 * EPL;  // emitters per workgroup
+    __shared__ float4 sq[kAdjWaves][kAdjTile];
+    __shared__ float2 sw[kAdjWaves][kAdjTile];
+    __shared__ double red[kAdjWaves][M];
 
-    double gx[EPL], gy[EPL], gz[EPL], gph[EPL], gam[EPL];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t ebase = (int64_t)blockIdx.x * M;
+
+    double acc[5][EPL];
 #pragma unroll
-    for (int i = 0; i < EPL; ++i) gx[i] = gy[i] = gz[i] = gph[i] = gam[i] = 0.0;
+    for (int i = 0; i < EPL; ++i) acc[0][i] = acc[1][i] = acc[2][i] = acc[3][i] = acc[4][i] = 0.0;
 
     const int t0 = blockIdx.y * tiles_per_split;
-    const int t1 = (t0 + tiles_per_split < ntiles) ? t0 + tiles_per_split : ntiles;
-    for (int t = t0; t < t1; ++t) {
-        const int tx0 = (t / tiles_y) * 16, ty0 = (t % tiles_y) * 16;
-        const int xe = min(tx0 + 16, g.n0) - 1, ye = min(ty0 + 16, g.n1) - 1;
-        const Frame c = pixel_frame(g, 0.5 * (tx0 + xe), 0.5 * (ty0 + ye));
-
-        __syncthreads();  // previous tile consumed
-        {
-            const int x = tx0 + (tid >> 4), y = ty0 + (tid & 15);
-            float4 q = make_float4(0.f, 0.f, 0.f, 1.f);
-            float2 w = make_float2(0.f, 0.f);
-            if (x < g.n0 && y < g.n1) {
-                const Frame f = pixel_frame(g, (double)x, (double)y);
-                q = make_float4((float)(-g.krev * (f.u[0] - c.u[0])), (float)(-g.krev * (f.u[1] - c.u[1])),
-                                (float)(-g.krev * (f.u[2] - c.u[2])), (float)(c.R / f.R));
-                const size_t idx = (size_t)x * g.n1 + y;
-                const float2 E = field[idx];
-                const double G = (double)dLdI[idx];
-                double wr = G * (double)E.x, wi = -G * (double)E.y;
-                if (g.nf) {
-                    double ph = g.krev * f.RmZ;
-                    ph -= rint(ph);
-                    double sn, cs;
-                    sincos(kTwoPi * ph, &sn, &cs);
-                    const double r2 = wr * cs - wi * sn;
-                    wi = wr * sn + wi * cs;
-                    wr = r2;
-                }
-                w = make_float2((float)wr, (float)wi);
+    const int t1 = min(t0 + tiles_per_split, ntiles);
+    const int iters = (t1 - t0 + kAdjWaves - 1) / kAdjWaves;  // the same for every wave
+    for (int it = 0; it < iters; ++it) {
+        const int t = t0 + it * kAdjWaves + wave;
+        const bool active = t < t1;  // wave-uniform
+        __syncthreads();             // this wave's previous tile consumed (uniform barrier count)
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < kAdjTile / 64; ++k) {
+                const int l = lane + 64 * k;
+                sq[wave][l] = pq[(size_t)t * kAdjTile + l];
+                sw[wave][l] = pw[(size_t)t * kAdjTile + l];
             }
-            sp0[tid] = q;
-            sp1[tid] = w;
         }
+        __syncthreads();
+        if (!active) continue;
 
-        float rx[EPL], ry[EPL], rz[EPL], rA[EPL], rW[EPL];
+        const Frame c = adj_tile_centre(g, t, tiles_y);
+        float rx[EPL], ry[EPL], rz[EPL], rA[EPL], rW[EPL], bx[EPL], by[EPL], bz[EPL];
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
-            const int64_t e = ebase + tid + (int64_t)i * kBlock;
+            const int64_t e = ebase + lane + 64 * i;
             if (e < n) {
                 const double* a4 = emit + 4 * e;
                 const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], 1.0);
@@ -85,20 +127,26 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
                 rz[i] = r.z;
                 rA[i] = r.A;
                 rW[i] = r.W;
+                // beta_cj in FP32 (|beta| ~ |r_j|/R; only its relative precision matters)
+                const float fx = (float)a4[0], fy = (float)a4[1], fz = (float)a4[2];
+                bx[i] = fx * r.iD - (float)c.rh[0] * r.sigma;
+                by[i] = fy * r.iD - (float)c.rh[1] * r.sigma;
+                bz[i] = fz * r.iD - (float)c.rh[2] * r.sigma;
             } else {
-                rx[i] = ry[i] = rz[i] = rA[i] = rW[i] = 0.f;
+                rx[i] = ry[i] = rz[i] = rA[i] = rW[i] = bx[i] = by[i] = bz[i] = 0.f;
             }
         }
-        __syncthreads();
 
-        float Sre[EPL], Sim[EPL], Vx[EPL], Vy[EPL], Vz[EPL], Lm[EPL];
+        float Sim[EPL], Vx[EPL], Vy[EPL], Vz[EPL], Sre[EPL], Lm[EPL];
 #pragma unroll
-        for (int i = 0; i < EPL; ++i) Sre[i] = Sim[i] = Vx[i] = Vy[i] = Vz[i] = Lm[i] = 0.f;
+        for (int i = 0; i < EPL; ++i) Sim[i] = Vx[i] = Vy[i] = Vz[i] = Sre[i] = Lm[i] = 0.f;
 
-#pragma unroll 2
-        for (int p = 0; p < kBlock; ++p) {
-            const float4 q = sp0[p];
-            const float2 w = sp1[p];
+        const float4* q4 = sq[wave];
+        const float2* w2 = sw[wave];
+#pragma unroll 1
+        for (int p = 0; p < kAdjTile; ++p) {
+            const float4 q = q4[p];
+            const float2 w = w2[p];
 #pragma unroll
             for (int i = 0; i < EPL; ++i) {
                 float tt = HAS_W ? fmaf(q.w, rW[i], rA[i]) : rA[i];
@@ -107,112 +155,173 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
                 tt = fmaf(q.z, rz[i], tt);
                 const float sn = __builtin_amdgcn_sinf(tt);
                 const float cs = __builtin_amdgcn_cosf(tt);
-                const float re = fmaf(w.x, cs, -w.y * sn);
                 const float im = fmaf(w.x, sn, w.y * cs);
-                Sre[i] += re;
                 Sim[i] += im;
                 Vx[i] = fmaf(im, q.x, Vx[i]);
                 Vy[i] = fmaf(im, q.y, Vy[i]);
                 Vz[i] = fmaf(im, q.z, Vz[i]);
-                if (HAS_W && !g.nf) Lm[i] = fmaf(im, q.w, Lm[i]);
+                if (LAM) Lm[i] = fmaf(im, q.w, Lm[i]);
+                if (AMP) Sre[i] += fmaf(w.x, cs, -w.y * sn);
             }
         }
 
-        // fold the tile into FP64 accumulators
+        // fold the tile into the FP64 accumulators:
+        // sum_p Im_p grad theta_pj = k [(beta - u_c) S + V / krev]
+        const double ik = 1.0 / g.krev;
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
-            const int64_t e = ebase + tid + (int64_t)i * kBlock;
-            if (e >= n) continue;
-            const double* a4 = emit + 4 * e;
-            const double ex = a4[0], ey = a4[1], ez = a4[2];
-            double bx = 0.0, by = 0.0, bz = 0.0;
-            if (g.nf) {
-                const double ucr = c.u[0] * ex + c.u[1] * ey + c.u[2] * ez;
-                const double s = ucr + ez;
-                const double hx = c.rh[1] * ez - c.rh[2] * ey;
-                const double hy = c.rh[2] * ex - c.rh[0] * ez;
-                const double hz = c.rh[0] * ey - c.rh[1] * ex;
-                const double h2 = hx * hx + hy * hy + hz * hz;
-                const double a = c.R - s;
-                const double D = sqrt(a * a + h2);
-                const double w = (a >= 0.0) ? h2 / (D + a) : D - a;
-                const double iD = 1.0 / D;
-                bx = (c.rh[0] * (w - s) + ex) * iD;
-                by = (c.rh[1] * (w - s) + ey) * iD;
-                bz = (c.rh[2] * (w - s) + ez) * iD;
+            const double S = (double)Sim[i];
+            acc[0][i] += g.k * (((double)bx[i] - c.u[0]) * S + (double)Vx[i] * ik);
+            acc[1][i] += g.k * (((double)by[i] - c.u[1]) * S + (double)Vy[i] * ik);
+            acc[2][i] += g.k * (((double)bz[i] - c.u[2]) * S + (double)Vz[i] * ik);
+            if (!g.nf && g.secondorder) {
+                // theta += k/(2 R_p)(x^2 + y^2) with R_c/R_p = lam (lam ~ 1 when W is folded)
+                const double L = (LAM ? (double)Lm[i] : S) * g.k / c.R;
+                acc[0][i] += L * (double)rx[i];
+                acc[1][i] += L * (double)ry[i];
             }
-            const double S_im = (double)Sim[i];
-            const double ik = 1.0 / g.krev;
-            // sum_p Im_p grad theta_pj = k [(-u_c + beta) S_im + V / krev]
-            gx[i] += g.k * ((bx - c.u[0]) * S_im + (double)Vx[i] * ik);
-            gy[i] += g.k * ((by - c.u[1]) * S_im + (double)Vy[i] * ik);
-            gz[i] += g.k * ((bz - c.u[2]) * S_im + (double)Vz[i] * ik);
-            if (HAS_W && !g.nf) {
-                // secondorder: theta += k/(2 R_p)(x^2+y^2), R_c/R_p = lam
-                const double L = (double)Lm[i] * g.k / c.R;
-                gx[i] += L * ex;
-                gy[i] += L * ey;
-            }
-            gph[i] += S_im;
-            gam[i] += (double)Sre[i];
+            acc[3][i] += S;
+            if (AMP) acc[4][i] += (double)Sre[i];
         }
     }
 
+    // reduce the four waves' sums (same emitters) and write this split's partials
 #pragma unroll
-    for (int i = 0; i < EPL; ++i) {
-        const int64_t e = ebase + tid + (int64_t)i * kBlock;
-        if (e >= n) continue;
-        const double a = (amp != nullptr) ? amp[e] : 1.0;
-        double* dst = partial + ((size_t)blockIdx.y * n + e) * 5;
-        dst[0] = -2.0 * a * gx[i];
-        dst[1] = -2.0 * a * gy[i];
-        dst[2] = -2.0 * a * gz[i];
-        dst[3] = -2.0 * a * gph[i];
-        dst[4] = 2.0 * gam[i];
+    for (int comp = 0; comp < 5; ++comp) {
+        if (comp == 4 && !AMP) break;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < EPL; ++i) red[wave][lane + 64 * i] = acc[comp][i];
+        __syncthreads();
+        for (int m = tid; m < M; m += kBlock) {
+            const int64_t e = ebase + m;
+            if (e >= n) continue;
+            double v = 0.0;
+#pragma unroll
+            for (int wv = 0; wv < kAdjWaves; ++wv) v += red[wv][m];
+            const double a = (amp != nullptr) ? amp[e] : 1.0;
+            partial[((size_t)blockIdx.y * n + e) * 5 + comp] = (comp == 4) ? 2.0 * v : -2.0 * a * v;
+        }
     }
 }
 
-__global__ void k_adjoint_finalize(const double* __restrict__ partial, int splits, int64_t n,
-                                   float* __restrict__ grad_pos, float* __restrict__ grad_phase,
-                                   float* __restrict__ grad_amp)
+// Sum of the partials over splits: one column per (emitter, component) value, four rows of
+// threads stride over the splits so that enough loads are in flight.
+__global__ __launch_bounds__(256) void k_adjoint_finalize(const double* __restrict__ partial, int splits,
+                                                          int64_t n, float* __restrict__ grad_pos,
+                                                          float* __restrict__ grad_phase,
+                                                          float* __restrict__ grad_amp)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    double v[5] = {0, 0, 0, 0, 0};
-    for (int s = 0; s < splits; ++s) {
-        const double* src = partial + ((size_t)s * n + e) * 5;
-#pragma unroll
-        for (int c = 0; c < 5; ++c) v[c] += src[c];
+    __shared__ double red[4][64];
+    const int col = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int64_t total = n * 5;
+    const int64_t f = (int64_t)blockIdx.x * 64 + col;  // e * 5 + component
+    double v0 = 0.0, v1 = 0.0;
+    if (f < total) {
+        const double* src = partial + f;
+        int s = row;
+        for (; s + 4 < splits; s += 8) {
+            v0 += src[(size_t)s * total];
+            v1 += src[(size_t)(s + 4) * total];
+        }
+        if (s < splits) v0 += src[(size_t)s * total];
     }
-    grad_pos[3 * e] = (float)v[0];
-    grad_pos[3 * e + 1] = (float)v[1];
-    grad_pos[3 * e + 2] = (float)v[2];
-    grad_phase[e] = (float)v[3];
-    if (grad_amp != nullptr) grad_amp[e] = (float)v[4];
+    red[row][col] = v0 + v1;
+    __syncthreads();
+    if (row == 0 && f < total) {
+        const double v = (red[0][col] + red[1][col]) + (red[2][col] + red[3][col]);
+        const int64_t e = f / 5;
+        const int c = (int)(f - 5 * e);
+        if (c < 3)
+            grad_pos[3 * e + c] = (float)v;
+        else if (c == 3)
+            grad_phase[e] = (float)v;
+        else if (grad_amp != nullptr)
+            grad_amp[e] = (float)v;
+    }
 }
 
-constexpr int kAdjEPL = 2;
+int adjoint_occupancy(bool w, bool lam, bool amp)
+{
+    static int cache[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int key = (w ? 4 : 0) | (lam ? 2 : 0) | (amp ? 1 : 0);
+    if (cache[key] > 0) return cache[key];
+    int nb = 0;
+    hipError_t e;
+    if (!w)
+        e = amp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<kAdjEPL, false, false, true>, kBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<kAdjEPL, false, false, false>, kBlock, 0);
+    else if (!lam)
+        e = amp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<kAdjEPL, true, false, true>, kBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<kAdjEPL, true, false, false>, kBlock, 0);
+    else
+        e = amp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<kAdjEPL, true, true, true>, kBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<kAdjEPL, true, true, false>, kBlock, 0);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    cache[key] = nb;
+    return nb;
+}
 
-hipError_t launch_adjoint(bool w, const Geom& g, const double* emit, const double* amp, int64_t n,
-                          const float2* field, const float* dLdI, int pixel_splits, double* partial,
+AdjointLayout adjoint_layout(const Geom& g, int64_t n, int num_cus, int blocks_per_cu)
+{
+    AdjointLayout L;
+    L.tiles_y = (g.n1 + 15) / 16;
+    L.ntiles = ((g.n0 + 15) / 16) * L.tiles_y;
+    L.egroups = (n + 64 * kAdjEPL - 1) / (64 * kAdjEPL);
+    // Each workgroup takes kAdjWaves * it tiles (it per wave).  Pick it to minimise the
+    // makespan rounds * it on num_cus * blocks_per_cu resident workgroups; on ties, fewer
+    // splits (less partial traffic).
+    const int64_t slots = (int64_t)num_cus * (blocks_per_cu > 0 ? blocks_per_cu : 1);
+    const int max_it = (L.ntiles + kAdjWaves - 1) / kAdjWaves;
+    int best_it = max_it;
+    int64_t best = INT64_MAX;
+    for (int it = 1; it <= max_it; ++it) {
+        const int64_t splits = (L.ntiles + kAdjWaves * it - 1) / (kAdjWaves * it);
+        if (it > 1 && splits == (L.ntiles + kAdjWaves * (it - 1) - 1) / (kAdjWaves * (it - 1))) continue;
+        const int64_t rounds = (L.egroups * splits + slots - 1) / slots;
+        const int64_t span = rounds * it;
+        if (span <= best) {
+            best = span;
+            best_it = it;
+        }
+    }
+    L.per = kAdjWaves * best_it;
+    L.splits = (L.ntiles + L.per - 1) / L.per;
+    L.pix_bytes = (size_t)L.ntiles * kAdjTile * (sizeof(float4) + sizeof(float2));
+    L.bytes = L.pix_bytes + (size_t)L.splits * (size_t)n * 5 * sizeof(double);
+    return L;
+}
+
+#define IDISIM_ADJ_LAUNCH(W, LM, A)                                                                        \
+    hipLaunchKernelGGL((k_adjoint<kAdjEPL, W, LM, A>), grid, dim3(kBlock), 0, st, g, emit, amp, n, pq, pw, \
+                       L.tiles_y, L.ntiles, L.per, partial)
+
+hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& L, const double* emit,
+                          const double* amp, int64_t n, const float2* field, const float* dLdI, void* scratch,
                           float* grad_pos, float* grad_phase, float* grad_amp, hipStream_t st)
 {
-    const int tiles_x = (g.n0 + 15) / 16, tiles_y = (g.n1 + 15) / 16;
-    const int ntiles = tiles_x * tiles_y;
-    if (pixel_splits < 1) pixel_splits = 1;
-    if (pixel_splits > ntiles) pixel_splits = ntiles;
-    const int per = (ntiles + pixel_splits - 1) / pixel_splits;
-    const int splits = (ntiles + per - 1) / per;
-    dim3 grid((unsigned)((n + kBlock * kAdjEPL - 1) / (kBlock * kAdjEPL)), (unsigned)splits);
-    if (w)
-        hipLaunchKernelGGL((k_adjoint<kAdjEPL, true>), grid, dim3(kBlock), 0, st, g, emit, amp, n, field, dLdI,
-                           tiles_y, ntiles, per, partial);
-    else
-        hipLaunchKernelGGL((k_adjoint<kAdjEPL, false>), grid, dim3(kBlock), 0, st, g, emit, amp, n, field, dLdI,
-                           tiles_y, ntiles, per, partial);
-    hipLaunchKernelGGL(k_adjoint_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial, splits,
+    float4* pq = static_cast<float4*>(scratch);
+    float2* pw = reinterpret_cast<float2*>(static_cast<char*>(scratch) + (size_t)L.ntiles * kAdjTile * sizeof(float4));
+    double* partial = reinterpret_cast<double*>(static_cast<char*>(scratch) + L.pix_bytes);
+    hipLaunchKernelGGL(k_adj_pixels, dim3((unsigned)L.ntiles), dim3(kAdjTile), 0, st, g, field, dLdI, L.tiles_y,
+                       pq, pw);
+    const dim3 grid((unsigned)L.egroups, (unsigned)L.splits);
+    const bool a = grad_amp != nullptr;
+    if (!w) {
+        if (a) IDISIM_ADJ_LAUNCH(false, false, true);
+        else IDISIM_ADJ_LAUNCH(false, false, false);
+    } else if (!lam) {
+        if (a) IDISIM_ADJ_LAUNCH(true, false, true);
+        else IDISIM_ADJ_LAUNCH(true, false, false);
+    } else {
+        if (a) IDISIM_ADJ_LAUNCH(true, true, true);
+        else IDISIM_ADJ_LAUNCH(true, true, false);
+    }
+    hipLaunchKernelGGL(k_adjoint_finalize, dim3((unsigned)((5 * n + 63) / 64)), dim3(256), 0, st, partial, L.splits,
                        n, grad_pos, grad_phase, grad_amp);
     return hipGetLastError();
 }
+
+#undef IDISIM_ADJ_LAUNCH
 
 }  // namespace idisim

@@ -107,6 +107,7 @@ struct SplitRec {
     float x, y, z, A;   // r'_cj and the reduced linear phase
     float W, amp;       // curvature / second-order coefficient, amplitude
     float sigma, s;     // sigma_cj and s_cj (nf)
+    float iD;           // 1 / D_cj (nf; used by the adjoint)
 };
 
 __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c, double ex,
@@ -116,7 +117,7 @@ __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c
     const double ucr = c.u[0] * ex + c.u[1] * ey + c.u[2] * ez;
     const double tau = -g.krev * ucr + ephi * kInv2Pi;
     const float fx = (float)ex, fy = (float)ey, fz = (float)ez;
-    float W = 0.f, sig = 0.f, am = (float)ea, s = 0.f;
+    float W = 0.f, sig = 0.f, am = (float)ea, s = 0.f, iD = 0.f;
     if (g.nf) {
         s = (float)(ucr + ez);
         const float rx = (float)c.rh[0], ry = (float)c.rh[1], rz = (float)c.rh[2];
@@ -130,6 +131,7 @@ __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c
         const float w = (a >= 0.f) ? h2 / (D + a) : D - a;
         W = (float)g.krev * w;
         sig = (s - w) / D;
+        iD = 1.f / D;
         if (g.scale) am = (float)ea * (R / D);
     } else if (g.secondorder) {
         W = (float)g.krev * (fx * fx + fy * fy) / (2.f * (float)c.R);
@@ -146,6 +148,7 @@ __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c
     r.amp = am;
     r.sigma = sig;
     r.s = s;
+    r.iD = iD;
     return r;
 }
 

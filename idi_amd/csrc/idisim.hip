@@ -905,18 +905,18 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
     if (bound > h->tol_rev)
         return fail(IDISIM_E_PRECISION, "idisim_adjoint: FP32-split error bound " + std::to_string(bound) +
                                             " rev exceeds the tolerance for this geometry");
-    const int ntiles = ((h->g.n0 + 15) / 16) * ((h->g.n1 + 15) / 16);
-    const int64_t egroups = (n + 511) / 512;
-    int64_t ps = (8LL * h->num_cus + egroups - 1) / egroups;
-    if (ps > ntiles) ps = ntiles;
-    if (ps < 1) ps = 1;
-    const int per = (int)((ntiles + ps - 1) / ps);
-    const int splits = (ntiles + per - 1) / per;
-    int rc = ensure_scratch(h, (size_t)splits * (size_t)n * 5 * sizeof(double));
+    Geom gl = h->g;
+    const bool curved = gl.nf || gl.secondorder;
+    const bool fold = curved && fold_error(h, 0, rho) <= 0.1 * h->tol_rev;
+    gl.foldw = fold ? 1 : 0;
+    const bool w = curved && !fold;
+    const bool lam = w && !gl.nf;
+    const AdjointLayout L = adjoint_layout(h->g, n, h->num_cus, adjoint_occupancy(w, lam, grad_amp != nullptr));
+    int rc = ensure_scratch(h, L.bytes);
     if (rc) return rc;
-    IDISIM_HIP(launch_adjoint(h->g.nf || h->g.secondorder, h->g, emitters, amplitudes, n,
-                              static_cast<const float2*>(field), dLdI, (int)ps, static_cast<double*>(h->scratch),
-                              grad_pos, grad_phase, grad_amp, st));
+    IDISIM_HIP(launch_adjoint(w, lam, gl, L, emitters, amplitudes, n, static_cast<const float2*>(field),
+                              dLdI, h->scratch, grad_pos, grad_phase, grad_amp, st));
+    const int splits = L.splits;
     h->plan.precision = IDISIM_PREC_FP32SPLIT;
     h->plan.tile = 0;
     h->plan.tile_rows = 16;

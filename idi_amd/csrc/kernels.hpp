@@ -36,10 +36,18 @@ hipError_t launch_batched_fp64(int v, const Geom& g, const double* emit, const d
                                hipStream_t st);
 int occupancy_batched_fp64(int v);
 
-// Adjoint (FP32-split).  pixel_splits > 1 writes double partials
-// [pixel_splits][n][5] to `partial` and a finalize kernel sums them.
-hipError_t launch_adjoint(bool w, const Geom& g, const double* emit, const double* amp, int64_t n,
-                          const float2* field, const float* dLdI, int pixel_splits, double* partial,
+// Adjoint (FP32-split) over 16 x 16 pixel tiles.  The scratch buffer holds the per-pixel
+// tile data (float4 + float2 per pixel) followed by double partials [splits][n][5].
+struct AdjointLayout {
+    int tiles_y, ntiles, per, splits;
+    int64_t egroups;
+    size_t pix_bytes, bytes;
+};
+AdjointLayout adjoint_layout(const Geom& g, int64_t n, int num_cus, int blocks_per_cu);
+int adjoint_occupancy(bool w, bool lam, bool amp);  // resident workgroups per CU
+// w: lam_p W_cj enters the phase (not folded); lam: far-field secondorder with w.
+hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& L, const double* emit,
+                          const double* amp, int64_t n, const float2* field, const float* dLdI, void* scratch,
                           float* grad_pos, float* grad_phase, float* grad_amp, hipStream_t st);
 
 hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out, hipStream_t st);

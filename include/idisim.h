@@ -30,6 +30,13 @@
 
 #include <stdint.h>
 
+/* The library is built with hidden visibility; only these entry points are exported. */
+#if defined(__GNUC__) || defined(__clang__)
+#define IDISIM_API __attribute__((visibility("default")))
+#else
+#define IDISIM_API
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -97,21 +104,21 @@ typedef struct idisim_plan {
 } idisim_plan;
 
 /* Library ABI version (IDISIM_VERSION). */
-int idisim_version(void);
+IDISIM_API int idisim_version(void);
 
 /* Thread-local message of the last error ("" if none). */
-const char *idisim_last_error(void);
+IDISIM_API const char *idisim_last_error(void);
 
 /* Create a handle on `device` for a fixed geometry and precision.
    Replaces the per-call kernel set-up of cuda.py:53-86 (flags, rotation,
    module compile) and the closure of cpu.py:6-40 (get_kernel). */
-int idisim_create(idisim_handle **out, const idisim_geometry *geom, int precision, int device);
+IDISIM_API int idisim_create(idisim_handle **out, const idisim_geometry *geom, int precision, int device);
 
-int idisim_destroy(idisim_handle *h);
+IDISIM_API int idisim_destroy(idisim_handle *h);
 
-int idisim_set_option(idisim_handle *h, int option, int64_t value);
+IDISIM_API int idisim_set_option(idisim_handle *h, int option, int64_t value);
 
-int idisim_get_plan(const idisim_handle *h, idisim_plan *out);
+IDISIM_API int idisim_get_plan(const idisim_handle *h, idisim_plan *out);
 
 /* One shot.  Replaces the kernels wfkernelf / wfkerneld (sim.cu:141-149) as
    launched by cuda.py:91-107, and the numba kernel(ret, atom) of cpu.py:68-107.
@@ -121,7 +128,7 @@ int idisim_get_plan(const idisim_handle *h, idisim_plan *out);
                  planner); <= 0 lets the library measure it (synchronises)
      out:        complex64 (FP32SPLIT) / complex128 (FP64) n0 x n1 for FIELD,
                  float32 / float64 n0 x n1 for INTENSITY, row-major, ret[x, y] */
-int idisim_forward(idisim_handle *h, const double *emitters, const double *amplitudes,
+IDISIM_API int idisim_forward(idisim_handle *h, const double *emitters, const double *amplitudes,
                    int64_t n, double extent, void *out, int out_kind, void *stream);
 
 /* Many shots with uneven emitter counts in one persistent launch (replayed as
@@ -130,7 +137,7 @@ int idisim_forward(idisim_handle *h, const double *emitters, const double *ampli
      emitters:   concatenated (sum n_s) x 4 float64
      offsets:    HOST array of nshots + 1 int64, offsets[0] = 0
      out:        nshots x n0 x n1 of the output kind/type of idisim_forward */
-int idisim_forward_batched(idisim_handle *h, const double *emitters, const double *amplitudes,
+IDISIM_API int idisim_forward_batched(idisim_handle *h, const double *emitters, const double *amplitudes,
                            const int64_t *offsets, int64_t nshots, double extent,
                            void *out, int out_kind, void *stream);
 
@@ -141,12 +148,12 @@ int idisim_forward_batched(idisim_handle *h, const double *emitters, const doubl
      grad_pos:   float32 n x 3  dL/d(x, y, z)
      grad_phase: float32 n      dL/dphi
      grad_amp:   float32 n      dL/da (may be NULL)                        */
-int idisim_adjoint(idisim_handle *h, const double *emitters, const double *amplitudes,
+IDISIM_API int idisim_adjoint(idisim_handle *h, const double *emitters, const double *amplitudes,
                    int64_t n, double extent, const void *field, const float *dLdI,
                    float *grad_pos, float *grad_phase, float *grad_amp, void *stream);
 
 /* Largest |r_j| of n emitters (device reduction; synchronises `stream`). */
-int idisim_extent(const double *emitters, int64_t n, double *extent_out, void *stream);
+IDISIM_API int idisim_extent(const double *emitters, int64_t n, double *extent_out, void *stream);
 
 /* Emitters drawn on the device (no reference counterpart): it replaces the
    host draws of simobj.sphere (simobj.py:119-130, util/random.py:39-51) and of
@@ -156,7 +163,7 @@ int idisim_extent(const double *emitters, int64_t n, double *extent_out, void *s
      positions: 1 = uniform in the ball |r - centre| <= radius, 0 = keep x, y, z
      phases:    1 = phi ~ U[0, 2 pi), 0 = keep phi
      centre:    HOST array of 3 doubles or NULL (origin)                    */
-int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, const double *centre,
+IDISIM_API int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, const double *centre,
                          int positions, int phases, double *out, void *stream);
 
 #ifdef __cplusplus

@@ -29,6 +29,21 @@ def test_exports_every_declared_symbol(lib):
     assert lib.idisim_version() == 1
 
 
+def test_exports_only_the_c_abi(lib):
+    """Internal C++ symbols are hidden: the dynamic symbol table holds exactly the header's functions."""
+    import shutil
+    import subprocess
+
+    from idi_amd import _lib
+
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True)
+    text_syms = sorted({ln.split()[-1] for ln in out.stdout.splitlines() if ln.split()[1:2] == ["T"]})
+    assert text_syms == _lib.declared_functions()
+
+
 def test_argument_errors_without_device(lib):
     from idi_amd import _lib
 
