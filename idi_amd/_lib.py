@@ -90,8 +90,10 @@ def load():
     lib.idisim_extent.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), vp]
     lib.idisim_generate_ball.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl,
                                          ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
+    lib.idisim_chargesharing.argtypes = [vp, i64, i64, dbl, dbl, ctypes.c_uint64, vp, vp]
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
-                 "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball"):
+                 "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
+                 "idisim_chargesharing"):
         getattr(lib, name).restype = ip
     if lib.idisim_version() != 1:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
@@ -140,6 +142,23 @@ def generate_ball(out, seed, rng_stream=0, radius=1.0, centre=(0.0, 0.0, 0.0), p
     check(lib.idisim_generate_ball(ctypes.c_uint64(int(seed) & (2**64 - 1)),
                                    ctypes.c_uint64(int(rng_stream) & (2**64 - 1)), int(out.shape[0]),
                                    float(radius), c, int(bool(positions)), int(bool(phases)), _ptr(out), _stream()))
+    return out
+
+
+def chargesharing(counts, out, sigma, gain_noise, seed):
+    """Charge sharing of a contiguous (n0, n1) float64 CUDA count image into `out` (same shape, no overlap)."""
+    import torch
+
+    _require_cuda()
+    lib = load()
+    for name, t in (("counts", counts), ("out", out)):
+        if t.dtype != torch.float64 or t.dim() != 2 or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError(f"{name} must be a contiguous 2-d float64 CUDA tensor")
+    if tuple(out.shape) != tuple(counts.shape):
+        raise ValueError("out must have the shape of counts")
+    check(lib.idisim_chargesharing(_ptr(counts), int(counts.shape[0]), int(counts.shape[1]), float(sigma),
+                                   float(gain_noise), ctypes.c_uint64(int(seed) & (2**64 - 1)), _ptr(out),
+                                   _stream()))
     return out
 
 

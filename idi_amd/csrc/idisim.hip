@@ -699,6 +699,26 @@ int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double r
     return IDISIM_OK;
 }
 
+int idisim_chargesharing(const double* counts, int64_t n0, int64_t n1, double sigma, double gain_noise,
+                         uint64_t seed, double* out, void* stream)
+{
+    g_err.clear();
+    if (n0 <= 0 || n1 <= 0 || n0 > (1 << 20) || n1 > (1 << 20) || n0 * n1 > ((int64_t)1 << 31))
+        return fail(IDISIM_E_INVALID, "idisim_chargesharing: image size out of range");
+    if (!counts || !out) return fail(IDISIM_E_INVALID, "idisim_chargesharing: NULL buffer");
+    const size_t bytes = (size_t)(n0 * n1) * sizeof(double);
+    const char* a = reinterpret_cast<const char*>(counts);
+    const char* b = reinterpret_cast<const char*>(out);
+    if (a < b + bytes && b < a + bytes) return fail(IDISIM_E_INVALID, "idisim_chargesharing: out overlaps counts");
+    if (sigma > 32.0)
+        return fail(IDISIM_E_INVALID, "idisim_chargesharing: sigma is limited to 32 pixels");
+    if (!std::isfinite(gain_noise) || gain_noise < 0.0)
+        return fail(IDISIM_E_INVALID, "idisim_chargesharing: gain_noise must be finite and >= 0");
+    IDISIM_HIP(launch_chargesharing(counts, (int)n0, (int)n1, sigma, gain_noise, seed, out,
+                                    static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
 // Decide FP32-split vs FP64 for a call; sets the plan's precision fields.
 static int choose_path(idisim_handle* h, double rho, std::vector<int>* tiles, bool* split_path)
 {

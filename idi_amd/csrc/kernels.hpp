@@ -56,4 +56,8 @@ hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out,
 hipError_t launch_gen_ball(uint64_t seed, uint64_t stream, int64_t n, double radius, const double* centre,
                            int positions, int phases, double* out, hipStream_t st);
 
+// Detector charge sharing / per-pixel gain noise (photons.hip).
+hipError_t launch_chargesharing(const double* counts, int n0, int n1, double s, double ns, uint64_t seed,
+                                double* out, hipStream_t st);
+
 }  // namespace idisim

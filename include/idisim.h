@@ -166,6 +166,19 @@ IDISIM_API int idisim_extent(const double *emitters, int64_t n, double *extent_o
 IDISIM_API int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, const double *centre,
                          int positions, int phases, double *out, void *stream);
 
+/* Charge sharing between neighbouring detector pixels (replaces chargesharing /
+   __putcharges, common.py:26-67).  Every count of `counts` (n0 x n1 float64,
+   device; truncated toward zero, non-positive values give no hits) is a hit at a
+   uniform random position in its pixel whose charge, a Gaussian of width `sigma`
+   pixels, is integrated over the (2w+1)^2 pixels around it, w = ceil(2 sigma);
+   charge leaving the detector is lost.  gain_noise > 0 scales each hit by
+   1 + gain_noise N(0,1).  sigma <= 0 (or NaN): out = counts (1 + gain_noise N(0,1))
+   per pixel, or a copy of counts.  Random numbers: Philox4x32-10 with key `seed`
+   (not numpy's stream).  out: n0 x n1 float64, device, must not overlap counts.
+   sigma is limited to 32 pixels.                                            */
+IDISIM_API int idisim_chargesharing(const double *counts, int64_t n0, int64_t n1, double sigma, double gain_noise,
+                                    uint64_t seed, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
