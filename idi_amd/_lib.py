@@ -91,9 +91,12 @@ def load():
     lib.idisim_generate_ball.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl,
                                          ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     lib.idisim_chargesharing.argtypes = [vp, i64, i64, dbl, dbl, ctypes.c_uint64, vp, vp]
+    lib.idisim_generate_balls.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl, i64, vp, vp, ip, ip, vp, vp]
+    lib.idisim_generate_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
                  "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
-                 "idisim_chargesharing"):
+                 "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss"):
         getattr(lib, name).restype = ip
     if lib.idisim_version() != 1:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
@@ -142,6 +145,45 @@ def generate_ball(out, seed, rng_stream=0, radius=1.0, centre=(0.0, 0.0, 0.0), p
     check(lib.idisim_generate_ball(ctypes.c_uint64(int(seed) & (2**64 - 1)),
                                    ctypes.c_uint64(int(rng_stream) & (2**64 - 1)), int(out.shape[0]),
                                    float(radius), c, int(bool(positions)), int(bool(phases)), _ptr(out), _stream()))
+    return out
+
+
+def _check_emitter_out(out):
+    import torch
+
+    if out.dtype != torch.float64 or out.dim() != 2 or out.shape[1] != 4 or not out.is_contiguous() or not out.is_cuda:
+        raise ValueError("out must be a contiguous (N, 4) float64 CUDA tensor")
+
+
+def generate_balls(out, seed, centres, counts, rng_stream=0, radius=1.0, positions=True, phases=True):
+    """Fill (N, 4) emitters: counts[b] of them uniform in the ball of `radius` about centres[b] (Philox4x32-10)."""
+    import torch
+
+    _require_cuda()
+    lib = load()
+    _check_emitter_out(out)
+    centres = torch.as_tensor(np.asarray(centres, dtype=np.float64).reshape(-1, 3)).to(out.device).contiguous()
+    counts = np.asarray(counts, dtype=np.int64).ravel()
+    if len(counts) != len(centres) or np.any(counts < 0) or int(counts.sum()) != out.shape[0]:
+        raise ValueError("counts must be non-negative, one per centre, summing to N")
+    offsets = torch.as_tensor(np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)).to(out.device)
+    check(lib.idisim_generate_balls(ctypes.c_uint64(int(seed) & (2**64 - 1)),
+                                    ctypes.c_uint64(int(rng_stream) & (2**64 - 1)), int(out.shape[0]), float(radius),
+                                    int(len(counts)), _ptr(centres), _ptr(offsets), int(bool(positions)),
+                                    int(bool(phases)), _ptr(out), _stream()))
+    return out
+
+
+def generate_gauss(out, seed, sigma, rot=None, rng_stream=0, positions=True, phases=True):
+    """Fill (N, 4) emitters with a Gaussian cloud of per-axis standard deviations `sigma`, rotated by `rot`."""
+    _require_cuda()
+    lib = load()
+    _check_emitter_out(out)
+    sig = (ctypes.c_double * 3)(*[float(v) for v in np.broadcast_to(np.asarray(sigma, dtype=np.float64), (3,))])
+    r = None if rot is None else (ctypes.c_double * 9)(*[float(v) for v in np.asarray(rot, dtype=np.float64).ravel()])
+    check(lib.idisim_generate_gauss(ctypes.c_uint64(int(seed) & (2**64 - 1)),
+                                    ctypes.c_uint64(int(rng_stream) & (2**64 - 1)), int(out.shape[0]), sig, r,
+                                    int(bool(positions)), int(bool(phases)), _ptr(out), _stream()))
     return out
 
 

@@ -699,6 +699,40 @@ int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double r
     return IDISIM_OK;
 }
 
+int idisim_generate_balls(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, int64_t nballs,
+                          const double* centres, const int64_t* offsets, int positions, int phases, double* out,
+                          void* stream)
+{
+    g_err.clear();
+    if (n < 0 || (n > 0 && !out)) return fail(IDISIM_E_INVALID, "idisim_generate_balls: bad arguments");
+    if (positions && n > 0 && (nballs < 1 || !centres || !offsets))
+        return fail(IDISIM_E_INVALID, "idisim_generate_balls: positions need nballs >= 1, centres and offsets");
+    if (!(radius >= 0.0) || !std::isfinite(radius))
+        return fail(IDISIM_E_INVALID, "idisim_generate_balls: radius must be finite and >= 0");
+    IDISIM_HIP(launch_gen_balls(seed, rng_stream, n, radius, nballs, centres, offsets, positions ? 1 : 0,
+                                phases ? 1 : 0, out, static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
+int idisim_generate_gauss(uint64_t seed, uint64_t rng_stream, int64_t n, const double* sigma, const double* rot,
+                          int positions, int phases, double* out, void* stream)
+{
+    g_err.clear();
+    if (n < 0 || (n > 0 && !out) || (positions && !sigma))
+        return fail(IDISIM_E_INVALID, "idisim_generate_gauss: bad arguments");
+    if (positions)
+        for (int k = 0; k < 3; ++k)
+            if (!(sigma[k] >= 0.0) || !std::isfinite(sigma[k]))
+                return fail(IDISIM_E_INVALID, "idisim_generate_gauss: sigma must be finite and >= 0");
+    if (rot)
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(rot[k])) return fail(IDISIM_E_INVALID, "idisim_generate_gauss: rot must be finite");
+    const double zero[3] = {0.0, 0.0, 0.0};
+    IDISIM_HIP(launch_gen_gauss(seed, rng_stream, n, sigma ? sigma : zero, rot, positions ? 1 : 0, phases ? 1 : 0,
+                                out, static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
 int idisim_chargesharing(const double* counts, int64_t n0, int64_t n1, double sigma, double gain_noise,
                          uint64_t seed, double* out, void* stream)
 {

@@ -56,6 +56,16 @@ hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out,
 hipError_t launch_gen_ball(uint64_t seed, uint64_t stream, int64_t n, double radius, const double* centre,
                            int positions, int phases, double* out, hipStream_t st);
 
+// Emitters in several balls (offsets: CSR over balls, device) and in a Gaussian cloud.
+struct Rot3 {
+    double m[9];
+};
+hipError_t launch_gen_balls(uint64_t seed, uint64_t stream, int64_t n, double radius, int64_t nballs,
+                            const double* centres, const int64_t* offsets, int positions, int phases, double* out,
+                            hipStream_t st);
+hipError_t launch_gen_gauss(uint64_t seed, uint64_t stream, int64_t n, const double* sigma, const double* rot,
+                            int positions, int phases, double* out, hipStream_t st);
+
 // Detector charge sharing / per-pixel gain noise (photons.hip).
 hipError_t launch_chargesharing(const double* counts, int n0, int n1, double s, double ns, uint64_t seed,
                                 double* out, hipStream_t st);

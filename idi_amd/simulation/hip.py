@@ -125,17 +125,44 @@ class _Staging:
 
 
 def _device_source(simobject):
-    """How the device can draw this object's emitters: 'ball' (positions and phases),
-    'phases' (fixed positions, random phases) or None (host draws)."""
+    """How the device can draw this object's emitters: 'ball' (simobj.sphere), 'balls'
+    (simobj.multisphere: host centres and counts), 'gauss' (simobj.gauss / gnorm with
+    rho = 2), each with fresh positions and phases; 'phases' (fixed positions, random
+    phases); or None (host draws)."""
     from . import simobj as _so
 
     rnd_pos = getattr(simobject, "rndPos", True)
     rnd_phase = getattr(simobject, "rndPhase", False)
     if not rnd_pos and rnd_phase:
         return "phases"
-    if type(simobject) is _so.sphere and rnd_pos and rnd_phase:
+    if not (rnd_pos and rnd_phase):
+        return None
+    if type(simobject) is _so.sphere:
         return "ball"
+    if type(simobject) is _so.multisphere:
+        return "balls"
+    if isinstance(simobject, _so.gnorm) and _np.all(simobject.rho == 2):
+        return "gauss"
     return None
+
+
+def _device_draw(source, simobject, out, seed):
+    """Draw the emitters of one image into `out` on the device; returns the extent for the planner
+    (0.0: let the library measure it)."""
+    if source == "ball":
+        _lib.generate_ball(out, seed, 0, simobject.r)
+        return float(simobject.r)
+    if source == "balls":
+        centres, counts = simobject.centres_counts()
+        _lib.generate_balls(out, seed, centres, counts, radius=simobject.rsphere)
+        used = _np.asarray(centres)[_np.asarray(counts) > 0]
+        return float(_np.linalg.norm(used, axis=1).max() + simobject.rsphere) if len(used) else 0.0
+    if source == "gauss":
+        sigma = _np.broadcast_to(simobject.fwhm / (2 * _np.sqrt(2 * _np.log(2))), (3,))
+        rot = simobject._rotmatrix if simobject._rotmatrix is not False else None
+        _lib.generate_gauss(out, seed, sigma, rot)
+        return 0.0
+    raise ValueError(source)
 
 
 def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", maximg=_np.inf, detangles=(0, 0),
@@ -148,10 +175,12 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
     stream; image i is copied back (or cloned on the device) after image i+1
     has been launched.
 
-    device_rng=True (extension): a ``simobj.sphere`` is drawn on the GPU
-    (idisim_generate_ball, Philox4x32-10, seed of image i drawn from
-    ``simobject.rng``), and an object with fixed positions (rndPos False) and
-    random phases gets its phases drawn there; nothing is copied per image.
+    device_rng=True (extension): ``simobj.sphere``, ``simobj.multisphere`` (ball
+    centres and counts still drawn on the host) and ``simobj.gauss`` / ``gnorm``
+    with rho = 2 are drawn on the GPU (idisim_generate_ball / _balls / _gauss,
+    Philox4x32-10, seed of image i drawn from ``simobject.rng``), and an object
+    with fixed positions (rndPos False) and random phases gets its phases drawn
+    there; no emitters are copied per image.
     """
     import torch
 
@@ -188,12 +217,11 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                     ext = _extent(atoms)
                 else:
                     seed = int(simobject.rng.integers(2**63))
-                    if source == "ball":
-                        _lib.generate_ball(st.d, seed, 0, simobject.r)
-                        ext = float(simobject.r)
-                    else:
+                    if source == "phases":
                         _lib.generate_ball(st.d, seed, 0, positions=False)
                         ext = fixed_extent
+                    else:
+                        ext = _device_draw(source, simobject, st.d, seed)
                     if amp is not None:
                         st.da.copy_(torch.from_numpy(amp))
                 h.forward(st.d, st.da if amp is not None else None, outs[slot], kind, ext)

@@ -220,7 +220,8 @@ class multisphere(simobj):
         return _pds(1.2 * float(_np.max(self.fwhm)), 2 * (self.rsphere + self.spacing), ndim=3, N=self._Nspheres,
                     rng=self.rng)
 
-    def updatePos(self):
+    def centres_counts(self):
+        """Ball centres (Poisson-disc sampled) and the number of emitters of each ball, summing to N."""
         centres = self._spherepos()
         if len(centres) == 0:
             centres = _np.zeros((1, 3))
@@ -233,6 +234,10 @@ class multisphere(simobj):
             _np.add.at(counts, ids, int(_np.sign(missing)))
             counts[counts < 0] = 0
             missing = self._N - counts.sum()
+        return centres, counts
+
+    def updatePos(self):
+        centres, counts = self.centres_counts()
         if self.rsphere > 0:
             pos = _rndSphere(self.rsphere, int(self._N), self.rng)
         else:

@@ -166,6 +166,25 @@ IDISIM_API int idisim_extent(const double *emitters, int64_t n, double *extent_o
 IDISIM_API int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, const double *centre,
                          int positions, int phases, double *out, void *stream);
 
+/* Emitters in several balls of one radius (no reference counterpart; it moves
+   the emitter draw of simobj.multisphere, simobj.py:230-262, onto the GPU; the
+   caller draws the ball centres and per-ball counts).  Emitter i is drawn as by
+   idisim_generate_ball with centre 0 and shifted by the centre of the ball b
+   with offsets[b] <= i < offsets[b + 1].
+     centres: nballs x 3 float64, DEVICE;  offsets: nballs + 1 int64, DEVICE,
+     non-decreasing, offsets[0] = 0, offsets[nballs] = n                      */
+IDISIM_API int idisim_generate_balls(uint64_t seed, uint64_t rng_stream, int64_t n, double radius, int64_t nballs,
+                                     const double *centres, const int64_t *offsets, int positions, int phases,
+                                     double *out, void *stream);
+
+/* Emitters in a Gaussian cloud (no reference counterpart; it moves the draw of
+   simobj.gauss / simobj.gnorm with rho = 2, simobj.py:154-184, onto the GPU):
+   x = sigma[0] z0, y = sigma[1] z1, z = sigma[2] z2 with standard normal z
+   (Box-Muller on Philox4x32-10), then p -> rot p if rot is not NULL.
+     sigma: HOST array of 3 standard deviations;  rot: HOST 3 x 3 row-major or NULL */
+IDISIM_API int idisim_generate_gauss(uint64_t seed, uint64_t rng_stream, int64_t n, const double *sigma,
+                                     const double *rot, int positions, int phases, double *out, void *stream);
+
 /* Charge sharing between neighbouring detector pixels (replaces chargesharing /
    __putcharges, common.py:26-67).  Every count of `counts` (n0 x n1 float64,
    device; truncated toward zero, non-positive values give no hits) is a hit at a

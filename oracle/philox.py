@@ -59,3 +59,42 @@ def generate_ball(n, seed, rng_stream=0, radius=1.0, centre=(0.0, 0.0, 0.0)):
     out[:, 2] = centre[2] + rr * st * np.cos(az)
     out[:, 3] = 2 * np.pi * u53(r1[:, 2], r1[:, 3])
     return out
+
+
+def _blocks(n, seed, rng_stream, word3):
+    i = np.arange(n, dtype=np.uint64)
+    ctr = np.zeros((n, 4), dtype=np.uint64)
+    ctr[:, 0] = i & MASK
+    ctr[:, 1] = i >> np.uint64(32)
+    ctr[:, 2] = np.uint64(int(rng_stream) & 0xFFFFFFFF)
+    ctr[:, 3] = word3
+    return philox4x32_10(ctr, (int(seed) & 0xFFFFFFFF, (int(seed) >> 32) & 0xFFFFFFFF))
+
+
+def generate_balls(seed, centres, counts, rng_stream=0, radius=1.0):
+    """(N, 4) emitters as idisim_generate_balls draws them: ball draws about the origin
+    shifted by the centre of each emitter's ball (counts[b] consecutive emitters per ball)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    out = generate_ball(int(counts.sum()), seed, rng_stream, radius)
+    out[:, :3] += np.repeat(np.asarray(centres, dtype=np.float64).reshape(-1, 3), counts, axis=0)
+    return out
+
+
+def _normal_pair(r):
+    rad = np.sqrt(-2.0 * np.log(1.0 - u53(r[:, 0], r[:, 1])))
+    ang = 2 * np.pi * u53(r[:, 2], r[:, 3])
+    return rad * np.cos(ang), rad * np.sin(ang)
+
+
+def generate_gauss(n, seed, sigma, rot=None, rng_stream=0):
+    """(n, 4) emitters as idisim_generate_gauss draws them (Box-Muller on blocks 0 and 1, phase from block 2)."""
+    z0, z1 = _normal_pair(_blocks(n, seed, rng_stream, 0))
+    z2, _ = _normal_pair(_blocks(n, seed, rng_stream, 1))
+    p = np.stack([z0, z1, z2], 1) * np.broadcast_to(np.asarray(sigma, dtype=np.float64), (3,))
+    if rot is not None:
+        p = p @ np.asarray(rot, dtype=np.float64).T
+    out = np.empty((n, 4))
+    out[:, :3] = p
+    r2 = _blocks(n, seed, rng_stream, 2)
+    out[:, 3] = 2 * np.pi * u53(r2[:, 0], r2[:, 1])
+    return out

@@ -57,3 +57,35 @@ def test_generate_ball_argument_errors(bad):
     rc = lib.idisim_generate_ball(ctypes.c_uint64(1), ctypes.c_uint64(0), n, bad.get("radius", 1.0), None, 1, 1,
                                   None if n <= 0 else ctypes.c_void_p(1), None)
     assert rc == -1
+
+
+def test_gauss_restatement_distribution():
+    from oracle.philox import generate_gauss
+
+    a = generate_gauss(200_000, seed=77, sigma=(1.0, 2.0, 0.5))
+    np.testing.assert_allclose(a[:, :3].std(0), (1.0, 2.0, 0.5), rtol=0.01)
+    assert np.abs(a[:, :3].mean(0)).max() < 0.01
+    k = np.mean((a[:, 0] / a[:, 0].std()) ** 4)
+    assert abs(k - 3) < 0.05  # Gaussian kurtosis
+    assert abs(np.corrcoef(a[:, 0], a[:, 1])[0, 1]) < 0.01
+    assert a[:, 3].min() >= 0 and a[:, 3].max() < 2 * np.pi
+    # rotation acts on the positions only; |p| is invariant
+    from idi_amd.util import rotation
+
+    M = rotation(0.3, -0.7, 1.1)
+    b = generate_gauss(1000, seed=77, sigma=1.0, rot=M)
+    c = generate_gauss(1000, seed=77, sigma=1.0)
+    np.testing.assert_allclose(b[:, :3], c[:, :3] @ M.T, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(np.linalg.norm(b[:, :3], axis=1), np.linalg.norm(c[:, :3], axis=1), rtol=1e-12)
+
+
+def test_balls_restatement_membership():
+    from oracle.philox import generate_balls
+
+    centres = np.array([[-40.0, 0, 0], [0, 0, 0], [50.0, 0, 0]])
+    counts = [100, 0, 250]
+    a = generate_balls(5, centres, counts, radius=15.0)
+    assert a.shape == (350, 4)
+    d0 = np.linalg.norm(a[:100, :3] - centres[0], axis=1)
+    d2 = np.linalg.norm(a[100:, :3] - centres[2], axis=1)
+    assert d0.max() <= 15.0 and d2.max() <= 15.0

@@ -51,3 +51,60 @@ def test_simulate_gen_device_rng(oracle_mod):
         seed = int(obj.rng.integers(2**63))
         at = generate_ball(3000, seed=seed, rng_stream=0, radius=0.05)
         assert rel_l2(imgs[i], oracle_mod.field_c(g, at)) <= 1e-4
+
+
+def test_device_balls_and_gauss_match_restatement():
+    import torch
+
+    from idi_amd import _lib
+    from idi_amd.util import rotation
+    from oracle.philox import generate_balls, generate_gauss
+
+    centres = np.array([[-0.04, 0.0, 0.0], [0.0, 0.01, 0.0], [0.05, 0.0, -0.02]])
+    counts = [1000, 0, 2501]
+    out = torch.empty((3501, 4), dtype=torch.float64, device="cuda")
+    _lib.generate_balls(out, seed=123, centres=centres, counts=counts, rng_stream=2, radius=0.015)
+    ref = generate_balls(123, centres, counts, rng_stream=2, radius=0.015)
+    got = out.cpu().numpy()
+    np.testing.assert_array_equal(got[:, 3], ref[:, 3])
+    assert np.abs(got[:, :3] - ref[:, :3]).max() <= 1e-17 * 8
+    M = rotation(0.2, 0.4, -0.3)
+    g = torch.empty((50_001, 4), dtype=torch.float64, device="cuda")
+    _lib.generate_gauss(g, seed=2**50 + 1, sigma=(0.01, 0.02, 0.005), rot=M, rng_stream=1)
+    ref = generate_gauss(50_001, seed=2**50 + 1, sigma=(0.01, 0.02, 0.005), rot=M, rng_stream=1)
+    got = g.cpu().numpy()
+    np.testing.assert_array_equal(got[:, 3], ref[:, 3])
+    np.testing.assert_allclose(got[:, :3], ref[:, :3], rtol=1e-13, atol=1e-18)
+
+
+@pytest.mark.parametrize("kind", ["multisphere", "gauss"])
+def test_simulate_gen_device_rng_shapes(oracle_mod, kind):
+    """simulate_gen(device_rng=True) for multisphere and gauss objects matches the oracle on the
+    emitters the device drew (regenerated with the restatement from the same seeds)."""
+    import oracle
+    from oracle.philox import generate_balls, generate_gauss
+
+    from idi_amd.simulation import hip, simobj
+
+    def make():
+        if kind == "multisphere":
+            o = simobj.multisphere(8048.0, Natoms=3000, rsphere=0.004, fwhm=0.03, spacing=0.001)
+        else:
+            o = simobj.gauss(8048.0, 3000, (0.02, 0.03, 0.01))
+        o.rng = np.random.default_rng(21)
+        return o
+
+    obj = make()
+    gen = hip.simulate_gen(obj, 24, 100.0, 1e5, K_UM, settings="single nf", maximg=2, device_rng=True,
+                           returngpu=True)
+    imgs = [g.cpu().numpy() for g in gen]
+    ref_obj = make()
+    g = oracle.Geometry(24, 24, 100.0, 1e5, K_UM, nf=True)
+    for i in range(2):
+        seed = int(ref_obj.rng.integers(2**63))
+        if kind == "multisphere":
+            centres, counts = ref_obj.centres_counts()
+            at = generate_balls(seed, centres, counts, radius=ref_obj.rsphere)
+        else:
+            at = generate_gauss(3000, seed, ref_obj.fwhm / (2 * np.sqrt(2 * np.log(2))))
+        assert rel_l2(imgs[i], oracle_mod.field_c(g, at)) <= 1e-4
