@@ -91,12 +91,15 @@ def load():
     lib.idisim_generate_ball.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl,
                                          ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     lib.idisim_chargesharing.argtypes = [vp, i64, i64, dbl, dbl, ctypes.c_uint64, vp, vp]
+    lib.idisim_corr_accumulate.argtypes = [vp, vp, i64, i64, vp, vp, vp]
+    lib.idisim_qcorr.argtypes = [vp, vp, i64, i64, dbl, dbl, dbl, i64, i64, vp, vp]
     lib.idisim_generate_balls.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl, i64, vp, vp, ip, ip, vp, vp]
     lib.idisim_generate_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
                  "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
-                 "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss"):
+                 "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss",
+                 "idisim_corr_accumulate", "idisim_qcorr"):
         getattr(lib, name).restype = ip
     if lib.idisim_version() != 1:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
@@ -184,6 +187,37 @@ def generate_gauss(out, seed, sigma, rot=None, rng_stream=0, positions=True, pha
     check(lib.idisim_generate_gauss(ctypes.c_uint64(int(seed) & (2**64 - 1)),
                                     ctypes.c_uint64(int(rng_stream) & (2**64 - 1)), int(out.shape[0]), sig, r,
                                     int(bool(positions)), int(bool(phases)), _ptr(out), _stream()))
+    return out
+
+
+def _check_cuda(name, t, dtype, shape=None):
+    if t.dtype != dtype or not t.is_contiguous() or not t.is_cuda or (shape is not None and tuple(t.shape) != shape):
+        raise ValueError(f"{name} must be a contiguous CUDA {dtype} tensor" + (f" of shape {shape}" if shape else ""))
+
+
+def corr_accumulate(corr, maskcorr, count, mean, m2):
+    """Masked division and Welford update of (mean, m2) with the count-th correlation (float32 tensors)."""
+    import torch
+
+    _require_cuda()
+    lib = load()
+    for name, t in (("corr", corr), ("maskcorr", maskcorr), ("mean", mean), ("m2", m2)):
+        _check_cuda(name, t, torch.float32, tuple(corr.shape))
+    check(lib.idisim_corr_accumulate(_ptr(corr), _ptr(maskcorr), int(corr.numel()), int(count), _ptr(mean), _ptr(m2),
+                                     _stream()))
+
+
+def qcorr(zd, img, xcenter, ycenter, z, qmax, maxdqz, out):
+    """q-corrected autocorrelation of one float32 image into out (2 maxdqz, 2 qmax, 2 qmax) float64."""
+    import torch
+
+    _require_cuda()
+    lib = load()
+    _check_cuda("zd", zd, torch.float32)
+    _check_cuda("img", img, torch.float32, tuple(zd.shape))
+    _check_cuda("out", out, torch.float64, (2 * int(maxdqz), 2 * int(qmax), 2 * int(qmax)))
+    check(lib.idisim_qcorr(_ptr(zd), _ptr(img), int(zd.shape[0]), int(zd.shape[1]), float(xcenter), float(ycenter),
+                           float(z), int(qmax), int(maxdqz), _ptr(out), _stream()))
     return out
 
 

@@ -733,6 +733,32 @@ int idisim_generate_gauss(uint64_t seed, uint64_t rng_stream, int64_t n, const d
     return IDISIM_OK;
 }
 
+int idisim_corr_accumulate(const float* corr, const float* maskcorr, int64_t n, int64_t count, float* mean,
+                           float* m2, void* stream)
+{
+    g_err.clear();
+    if (n < 0 || count < 1 || (n > 0 && (!corr || !maskcorr || !mean || !m2)))
+        return fail(IDISIM_E_INVALID, "idisim_corr_accumulate: bad arguments");
+    IDISIM_HIP(launch_corr_accumulate(corr, maskcorr, n, count, mean, m2, static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
+int idisim_qcorr(const float* zd, const float* img, int64_t nr, int64_t nc, double xcenter, double ycenter, double z,
+                 int64_t qmax, int64_t maxdqz, double* out, void* stream)
+{
+    g_err.clear();
+    if (!zd || !img || !out) return fail(IDISIM_E_INVALID, "idisim_qcorr: NULL buffer");
+    if (nr < 1 || nc < 1 || nr > 65535 || nc > (1 << 20))
+        return fail(IDISIM_E_INVALID, "idisim_qcorr: image size out of range");
+    if (qmax < 1 || qmax > 1024 || maxdqz < 1 || maxdqz > (1 << 20))
+        return fail(IDISIM_E_INVALID, "idisim_qcorr: qmax must be in [1, 1024] and maxdqz >= 1");
+    if (!std::isfinite(xcenter) || !std::isfinite(ycenter) || !std::isfinite(z))
+        return fail(IDISIM_E_INVALID, "idisim_qcorr: centre and z must be finite");
+    IDISIM_HIP(launch_qcorr(zd, img, (int)nr, (int)nc, (float)xcenter, (float)ycenter, (float)z, (int)qmax,
+                            (int)maxdqz, out, static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
 int idisim_chargesharing(const double* counts, int64_t n0, int64_t n1, double sigma, double gain_noise,
                          uint64_t seed, double* out, void* stream)
 {

@@ -66,6 +66,12 @@ hipError_t launch_gen_balls(uint64_t seed, uint64_t stream, int64_t n, double ra
 hipError_t launch_gen_gauss(uint64_t seed, uint64_t stream, int64_t n, const double* sigma, const double* rot,
                             int positions, int phases, double* out, hipStream_t st);
 
+// Correlators (correlate.hip).
+hipError_t launch_corr_accumulate(const float* corr, const float* maskcorr, int64_t n, int64_t count, float* mean,
+                                  float* m2, hipStream_t st);
+hipError_t launch_qcorr(const float* zd, const float* img, int nr, int nc, float xc, float yc, float cz, int qmax,
+                        int maxdqz, double* out, hipStream_t st);
+
 // Detector charge sharing / per-pixel gain noise (photons.hip).
 hipError_t launch_chargesharing(const double* counts, int n0, int n1, double s, double ns, uint64_t seed,
                                 double* out, hipStream_t st);

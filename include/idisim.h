@@ -198,6 +198,22 @@ IDISIM_API int idisim_generate_gauss(uint64_t seed, uint64_t rng_stream, int64_t
 IDISIM_API int idisim_chargesharing(const double *counts, int64_t n0, int64_t n1, double sigma, double gain_noise,
                                     uint64_t seed, double *out, void *stream);
 
+/* One streaming step of the 2-d FFT correlator (replaces the division and
+   running statistics of correlator2d.add, cusimple.py:80-107): per element
+   c = corr / maskcorr where maskcorr > 0.5, else 0; count == 1 starts mean = c,
+   m2 = 0, later counts update mean += (c - mean) / count and
+   m2 += (c - mean_old)(c - mean_new).  All arrays: n float32, device.        */
+IDISIM_API int idisim_corr_accumulate(const float *corr, const float *maskcorr, int64_t n, int64_t count,
+                                      float *mean, float *m2, void *stream);
+
+/* q-corrected autocorrelation of one nr x nc image (replaces the kernel and the
+   assemble step of cucor.corrfunction, cucor.py:9-133).  zd: nr x nc float32
+   z / |(r - xcenter, c - ycenter, z)|, img: nr x nc float32, both device.
+   out: (2 maxdqz) x (2 qmax) x (2 qmax) float64, device, overwritten.
+   Requires 1 <= qmax <= 1024, maxdqz >= 1.                                     */
+IDISIM_API int idisim_qcorr(const float *zd, const float *img, int64_t nr, int64_t nc, double xcenter,
+                            double ycenter, double z, int64_t qmax, int64_t maxdqz, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,7 +74,7 @@ def test_device_balls_and_gauss_match_restatement():
     ref = generate_gauss(50_001, seed=2**50 + 1, sigma=(0.01, 0.02, 0.005), rot=M, rng_stream=1)
     got = g.cpu().numpy()
     np.testing.assert_array_equal(got[:, 3], ref[:, 3])
-    np.testing.assert_allclose(got[:, :3], ref[:, :3], rtol=1e-13, atol=1e-18)
+    np.testing.assert_allclose(got[:, :3], ref[:, :3], rtol=1e-13, atol=1e-16)  # 1 ulp at 1e-2 after cancellation
 
 
 @pytest.mark.parametrize("kind", ["multisphere", "gauss"])
