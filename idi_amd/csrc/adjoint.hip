@@ -13,11 +13,15 @@
 // so per (tile, emitter) only S = sum_p Im(w_p e^{i theta}) and V = sum_p Im(.) dq_p
 // (3 floats) are needed in FP32 (plus sum Re(.) for dL/da); they are folded into FP64
 // accumulators with the FP64 tile-centre vector u_c after every tile.
+// With w'_p = m_p e^{2 pi i psi_p}, Im(w'_p e^{2 pi i tau}) = m_p sin(2 pi (tau + psi_p)):
+// one transcendental per pair, and m_p, m_p dq_p are per-pixel table entries, so
+// S and V are four FMAs on that sine (the cosine is needed for dL/da only).
 //
 // Kernels:
 //   k_adj_pixels  once per call: per pixel of each 16 x 16 tile, dq_p and lam_p = R_c/R_p
 //                 relative to the tile centre and w'_p = w_p e^{2 pi i Phi_p} (the per-pixel
-//                 near-field phase, FP64), laid out tile-major for LDS staging.
+//                 near-field phase, FP64) as (dq, psi), m (dq, 1), (lam, m lam), laid out
+//                 tile-major for LDS staging.
 //   k_adjoint     emitter-parallel: a workgroup owns 64*EPL emitters, shared by its four
 //                 waves; each wave streams its own pixel tiles (tile t0 + 4 it + wave) through
 //                 a private LDS slot, so the 4 waves cover 4 tiles per step.  The per-wave
@@ -44,17 +48,19 @@ __device__ inline Frame adj_tile_centre(const Geom& g, int t, int tiles_y)
 
 __global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* __restrict__ field,
                                                          const float* __restrict__ dLdI, int tiles_y,
-                                                         float4* __restrict__ pq, float2* __restrict__ pw)
+                                                         float4* __restrict__ pa, float4* __restrict__ pb,
+                                                         float2* __restrict__ pc)
 {
     const int t = blockIdx.x, l = threadIdx.x;
     const int x = (t / tiles_y) * 16 + (l >> 4), y = (t % tiles_y) * 16 + (l & 15);
-    float4 q = make_float4(0.f, 0.f, 0.f, 1.f);
-    float2 w = make_float2(0.f, 0.f);
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 c2 = make_float2(1.f, 0.f);
     if (x < g.n0 && y < g.n1) {
         const Frame c = adj_tile_centre(g, t, tiles_y);
         const Frame f = pixel_frame(g, (double)x, (double)y);
-        q = make_float4((float)(-g.krev * (f.u[0] - c.u[0])), (float)(-g.krev * (f.u[1] - c.u[1])),
-                        (float)(-g.krev * (f.u[2] - c.u[2])), (float)(c.R / f.R));
+        const double dqx = -g.krev * (f.u[0] - c.u[0]), dqy = -g.krev * (f.u[1] - c.u[1]);
+        const double dqz = -g.krev * (f.u[2] - c.u[2]), lam = c.R / f.R;
         const size_t idx = (size_t)x * g.n1 + y;
         const float2 E = field[idx];
         const double G = (double)dLdI[idx];
@@ -68,10 +74,15 @@ __global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* _
             wi = wr * sn + wi * cs;
             wr = r2;
         }
-        w = make_float2((float)wr, (float)wi);
+        const double m = sqrt(wr * wr + wi * wi);
+        const double psi = (m > 0.0) ? atan2(wi, wr) * kInv2Pi : 0.0;
+        a = make_float4((float)dqx, (float)dqy, (float)dqz, (float)psi);
+        b = make_float4((float)(m * dqx), (float)(m * dqy), (float)(m * dqz), (float)m);
+        c2 = make_float2((float)lam, (float)(m * lam));
     }
-    pq[(size_t)t * kAdjTile + l] = q;
-    pw[(size_t)t * kAdjTile + l] = w;
+    pa[(size_t)t * kAdjTile + l] = a;
+    pb[(size_t)t * kAdjTile + l] = b;
+    pc[(size_t)t * kAdjTile + l] = c2;
 }
 
 // HAS_W: lam_p W_cj enters the phase (nf or secondorder, not folded into A).
@@ -80,14 +91,22 @@ __global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* _
 template <int EPL, bool HAS_W, bool LAM, bool AMP>
 __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __restrict__ emit,
                                                     const double* __restrict__ amp, int64_t n,
-                                                    const float4* __restrict__ pq, const float2* __restrict__ pw,
-                                                    int tiles_y, int ntiles, int tiles_per_split,
-                                                    double* __restrict__ partial)
+                                                    const float4* __restrict__ pa, const float4* __restrict__ pb,
+                                                    const float2* __restrict__ pc, int tiles_y, int ntiles,
+                                                    int tiles_per_split, double* __restrict__ partial)
 {
     constexpr int M = 64 * EPL;  // emitters per workgroup
-    __shared__ float4 sq[kAdjWaves][kAdjTile];
-    __shared__ float2 sw[kAdjWaves][kAdjTile];
-    __shared__ double red[kAdjWaves][M];
+    constexpr bool USE_LAM = HAS_W || LAM;
+    // the cross-wave reduction after the tile loop reuses the staging area of sa
+    union StageOrReduce {
+        float4 a[kAdjWaves][kAdjTile];
+        double red[kAdjWaves][M];
+    };
+    __shared__ StageOrReduce u;
+    __shared__ float4 sb[kAdjWaves][kAdjTile];
+    __shared__ float2 sc[kAdjWaves][USE_LAM ? kAdjTile : 1];
+    float4(&sa)[kAdjWaves][kAdjTile] = u.a;
+    double(&red)[kAdjWaves][M] = u.red;
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t ebase = (int64_t)blockIdx.x * M;
@@ -107,8 +126,9 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
 #pragma unroll
             for (int k = 0; k < kAdjTile / 64; ++k) {
                 const int l = lane + 64 * k;
-                sq[wave][l] = pq[(size_t)t * kAdjTile + l];
-                sw[wave][l] = pw[(size_t)t * kAdjTile + l];
+                sa[wave][l] = pa[(size_t)t * kAdjTile + l];
+                sb[wave][l] = pb[(size_t)t * kAdjTile + l];
+                if (USE_LAM) sc[wave][l] = pc[(size_t)t * kAdjTile + l];
             }
         }
         __syncthreads();
@@ -141,27 +161,28 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
 #pragma unroll
         for (int i = 0; i < EPL; ++i) Sim[i] = Vx[i] = Vy[i] = Vz[i] = Sre[i] = Lm[i] = 0.f;
 
-        const float4* q4 = sq[wave];
-        const float2* w2 = sw[wave];
+        const float4* a4p = sa[wave];
+        const float4* b4p = sb[wave];
 #pragma unroll 1
         for (int p = 0; p < kAdjTile; ++p) {
-            const float4 q = q4[p];
-            const float2 w = w2[p];
+            const float4 a = a4p[p];  // dq, psi
+            const float4 b = b4p[p];  // m dq, m
+            float2 c2 = make_float2(1.f, 0.f);
+            if (USE_LAM) c2 = sc[wave][p];  // lam, m lam
 #pragma unroll
             for (int i = 0; i < EPL; ++i) {
-                float tt = HAS_W ? fmaf(q.w, rW[i], rA[i]) : rA[i];
-                tt = fmaf(q.x, rx[i], tt);
-                tt = fmaf(q.y, ry[i], tt);
-                tt = fmaf(q.z, rz[i], tt);
+                float tt = rA[i] + a.w;
+                if (HAS_W) tt = fmaf(c2.x, rW[i], tt);
+                tt = fmaf(a.x, rx[i], tt);
+                tt = fmaf(a.y, ry[i], tt);
+                tt = fmaf(a.z, rz[i], tt);
                 const float sn = __builtin_amdgcn_sinf(tt);
-                const float cs = __builtin_amdgcn_cosf(tt);
-                const float im = fmaf(w.x, sn, w.y * cs);
-                Sim[i] += im;
-                Vx[i] = fmaf(im, q.x, Vx[i]);
-                Vy[i] = fmaf(im, q.y, Vy[i]);
-                Vz[i] = fmaf(im, q.z, Vz[i]);
-                if (LAM) Lm[i] = fmaf(im, q.w, Lm[i]);
-                if (AMP) Sre[i] += fmaf(w.x, cs, -w.y * sn);
+                Sim[i] = fmaf(b.w, sn, Sim[i]);
+                Vx[i] = fmaf(b.x, sn, Vx[i]);
+                Vy[i] = fmaf(b.y, sn, Vy[i]);
+                Vz[i] = fmaf(b.z, sn, Vz[i]);
+                if (LAM) Lm[i] = fmaf(c2.y, sn, Lm[i]);
+                if (AMP) Sre[i] = fmaf(b.w, __builtin_amdgcn_cosf(tt), Sre[i]);
             }
         }
 
@@ -185,7 +206,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
         }
     }
 
-    // reduce the four waves' sums (same emitters) and write this split's partials
+    // reduce the four waves' sums (same emitters) and write this split's partials; the
+    // first barrier also ends every wave's use of sa
 #pragma unroll
     for (int comp = 0; comp < 5; ++comp) {
         if (comp == 4 && !AMP) break;
@@ -287,24 +309,26 @@ AdjointLayout adjoint_layout(const Geom& g, int64_t n, int num_cus, int blocks_p
     }
     L.per = kAdjWaves * best_it;
     L.splits = (L.ntiles + L.per - 1) / L.per;
-    L.pix_bytes = (size_t)L.ntiles * kAdjTile * (sizeof(float4) + sizeof(float2));
+    L.pix_bytes = (size_t)L.ntiles * kAdjTile * (2 * sizeof(float4) + sizeof(float2));
     L.bytes = L.pix_bytes + (size_t)L.splits * (size_t)n * 5 * sizeof(double);
     return L;
 }
 
 #define IDISIM_ADJ_LAUNCH(W, LM, A)                                                                        \
-    hipLaunchKernelGGL((k_adjoint<kAdjEPL, W, LM, A>), grid, dim3(kBlock), 0, st, g, emit, amp, n, pq, pw, \
+    hipLaunchKernelGGL((k_adjoint<kAdjEPL, W, LM, A>), grid, dim3(kBlock), 0, st, g, emit, amp, n, pa, pb, pc, \
                        L.tiles_y, L.ntiles, L.per, partial)
 
 hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& L, const double* emit,
                           const double* amp, int64_t n, const float2* field, const float* dLdI, void* scratch,
                           float* grad_pos, float* grad_phase, float* grad_amp, hipStream_t st)
 {
-    float4* pq = static_cast<float4*>(scratch);
-    float2* pw = reinterpret_cast<float2*>(static_cast<char*>(scratch) + (size_t)L.ntiles * kAdjTile * sizeof(float4));
+    const size_t np = (size_t)L.ntiles * kAdjTile;
+    float4* pa = static_cast<float4*>(scratch);
+    float4* pb = pa + np;
+    float2* pc = reinterpret_cast<float2*>(pb + np);
     double* partial = reinterpret_cast<double*>(static_cast<char*>(scratch) + L.pix_bytes);
     hipLaunchKernelGGL(k_adj_pixels, dim3((unsigned)L.ntiles), dim3(kAdjTile), 0, st, g, field, dLdI, L.tiles_y,
-                       pq, pw);
+                       pa, pb, pc);
     const dim3 grid((unsigned)L.egroups, (unsigned)L.splits);
     const bool a = grad_amp != nullptr;
     if (!w) {

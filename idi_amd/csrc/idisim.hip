@@ -981,7 +981,9 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
         int rc = measure_extent(h, emitters, n, st, &rho);
         if (rc) return rc;
     }
-    const double bound = plan_bound(h, 0, rho);  // the adjoint streams 16 x 16 pixel tiles
+    // the adjoint streams 16 x 16 pixel tiles; it adds the per-pixel phase psi_p (FP32,
+    // |psi| <= 1/2) to the reduced phase: one more rounding at |tau + psi| <= 1 rev
+    const double bound = plan_bound(h, 0, rho) + 0x1p-23;
     if (bound > h->tol_rev)
         return fail(IDISIM_E_PRECISION, "idisim_adjoint: FP32-split error bound " + std::to_string(bound) +
                                             " rev exceeds the tolerance for this geometry");

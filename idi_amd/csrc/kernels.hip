@@ -195,19 +195,41 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
     }
 }
 
-__global__ void k_finalize_split(Geom g, const float4* __restrict__ partial, int splits, void* out,
-                                 int kind)
+// FP64 combine of the split partials: a workgroup owns 64 pixels, its four rows of
+// threads stride over the splits (several loads in flight per pixel) and the rows are
+// summed through LDS.
+__global__ __launch_bounds__(256) void k_finalize_split(Geom g, const float4* __restrict__ partial, int splits,
+                                                        void* out, int kind)
 {
+    __shared__ double red[2][4][64];
     const size_t npix = (size_t)g.n0 * g.n1;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= npix) return;
-    double re = 0.0, im = 0.0;
-    for (int s = 0; s < splits; ++s) {
-        const float4 v = partial[(size_t)s * npix + idx];
-        re += (double)v.x - (double)v.z;
-        im += (double)v.y - (double)v.w;
+    const int col = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const size_t idx = (size_t)blockIdx.x * 64 + col;
+    double re0 = 0.0, im0 = 0.0, re1 = 0.0, im1 = 0.0;
+    if (idx < npix) {
+        int s = row;
+        for (; s + 4 < splits; s += 8) {
+            const float4 a = partial[(size_t)s * npix + idx];
+            const float4 b = partial[(size_t)(s + 4) * npix + idx];
+            re0 += (double)a.x - (double)a.z;
+            im0 += (double)a.y - (double)a.w;
+            re1 += (double)b.x - (double)b.z;
+            im1 += (double)b.y - (double)b.w;
+        }
+        if (s < splits) {
+            const float4 a = partial[(size_t)s * npix + idx];
+            re0 += (double)a.x - (double)a.z;
+            im0 += (double)a.y - (double)a.w;
+        }
     }
-    store_split(g, (int)(idx / g.n1), (int)(idx % g.n1), re, im, out, kind, idx);
+    red[0][row][col] = re0 + re1;
+    red[1][row][col] = im0 + im1;
+    __syncthreads();
+    if (row == 0 && idx < npix) {
+        const double re = (red[0][0][col] + red[0][1][col]) + (red[0][2][col] + red[0][3][col]);
+        const double im = (red[1][0][col] + red[1][1][col]) + (red[1][2][col] + red[1][3][col]);
+        store_split(g, (int)(idx / g.n1), (int)(idx % g.n1), re, im, out, kind, idx);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -506,8 +528,8 @@ hipError_t launch_finalize_split(const Geom& g, const float4* partial, int split
                                  hipStream_t st)
 {
     const size_t npix = (size_t)g.n0 * g.n1;
-    hipLaunchKernelGGL(k_finalize_split, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, g, partial,
-                       splits, out, kind);
+    hipLaunchKernelGGL(k_finalize_split, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, st, g, partial, splits,
+                       out, kind);
     return hipGetLastError();
 }
 
