@@ -91,6 +91,7 @@ def load():
     lib.idisim_generate_ball.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl,
                                          ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     lib.idisim_chargesharing.argtypes = [vp, i64, i64, dbl, dbl, ctypes.c_uint64, vp, vp]
+    lib.idisim_time_forward.argtypes = [vp, vp, vp, i64, dbl, dbl, vp, vp]
     lib.idisim_corr_accumulate.argtypes = [vp, vp, i64, i64, vp, vp, vp]
     lib.idisim_qcorr.argtypes = [vp, vp, i64, i64, dbl, dbl, dbl, i64, i64, vp, vp]
     lib.idisim_generate_balls.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl, i64, vp, vp, ip, ip, vp, vp]
@@ -99,7 +100,7 @@ def load():
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
                  "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
                  "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss",
-                 "idisim_corr_accumulate", "idisim_qcorr"):
+                 "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward"):
         getattr(lib, name).restype = ip
     if lib.idisim_version() != 1:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
@@ -348,6 +349,23 @@ class Handle:
         check(self._lib.idisim_adjoint(self._h, _ptr(emit), _ptr(amp), int(n), float(extent), _ptr(field),
                                        _ptr(dLdI), _ptr(gpos), _ptr(gph), _ptr(gamp), _stream()))
         return gpos, gph, gamp
+
+    def time_forward(self, emit, times, c, tau, out=None):
+        """Time-integrated intensity (n0, n1) float64 of decaying emitters excited at `times`."""
+        import torch
+
+        n = emit.shape[0]
+        if emit.dtype != torch.float64 or emit.dim() != 2 or emit.shape[1] != 4 or not emit.is_contiguous():
+            raise ValueError("emit must be a contiguous (N, 4) float64 CUDA tensor")
+        if times.dtype != torch.float64 or tuple(times.shape) != (n,) or not times.is_contiguous():
+            raise ValueError("times must be a contiguous (N,) float64 CUDA tensor")
+        if out is None:
+            out = torch.empty(self.shape, dtype=torch.float64, device=emit.device)
+        elif out.dtype != torch.float64 or tuple(out.shape) != self.shape or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 tensor of the detector shape")
+        check(self._lib.idisim_time_forward(self._h, _ptr(emit), _ptr(times), int(n), float(c), float(tau), _ptr(out),
+                                            _stream()))
+        return out
 
     def _check_out(self, out, kind, nshots):
         import torch

@@ -733,6 +733,25 @@ int idisim_generate_gauss(uint64_t seed, uint64_t rng_stream, int64_t n, const d
     return IDISIM_OK;
 }
 
+int idisim_time_forward(idisim_handle* h, const double* emitters, const double* times, int64_t n, double c,
+                        double tau, double* out, void* stream)
+{
+    g_err.clear();
+    if (!h) return fail(IDISIM_E_INVALID, "idisim_time_forward: NULL handle");
+    if (n < 0 || n >= ((int64_t)1 << 31) || (n > 0 && (!emitters || !times)) || !out)
+        return fail(IDISIM_E_INVALID, "idisim_time_forward: bad arguments");
+    if (!(c > 0.0) || !std::isfinite(c) || !(tau > 0.0) || !std::isfinite(tau))
+        return fail(IDISIM_E_INVALID, "idisim_time_forward: c and tau must be finite and > 0");
+    DeviceGuard guard(h->device);
+    const TimeLayout L = time_layout(h->g, n, h->precision == IDISIM_PREC_FP64, (size_t)1 << 30);
+    if (n > 0) {
+        int rc = ensure_scratch(h, L.bytes);
+        if (rc) return rc;
+    }
+    IDISIM_HIP(launch_time(h->g, L, emitters, times, n, c, tau, h->scratch, out, static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
 int idisim_corr_accumulate(const float* corr, const float* maskcorr, int64_t n, int64_t count, float* mean,
                            float* m2, void* stream)
 {

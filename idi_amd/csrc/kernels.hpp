@@ -72,6 +72,17 @@ hipError_t launch_corr_accumulate(const float* corr, const float* maskcorr, int6
 hipError_t launch_qcorr(const float* zd, const float* img, int nr, int nc, float xc, float yc, float cz, int qmax,
                         int maxdqz, double* out, hipStream_t st);
 
+// Time-dependent simulation (timesim.hip): pixels are processed in batches of np; the scratch
+// holds keys (2x), indices (2x), amplitudes, segment offsets and the sort's temporary storage.
+struct TimeLayout {
+    int np = 1;
+    bool fp64 = false;
+    size_t keys_bytes = 0, idx_bytes = 0, amp_bytes = 0, off_bytes = 0, sort_bytes = 0, bytes = 0;
+};
+TimeLayout time_layout(const Geom& g, int64_t n, bool fp64_keys, size_t budget);
+hipError_t launch_time(const Geom& g, const TimeLayout& L, const double* emit, const double* times, int64_t n,
+                       double c, double tau, void* scratch, double* out, hipStream_t st);
+
 // Detector charge sharing / per-pixel gain noise (photons.hip).
 hipError_t launch_chargesharing(const double* counts, int n0, int n1, double s, double ns, uint64_t seed,
                                 double* out, hipStream_t st);

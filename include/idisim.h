@@ -198,6 +198,19 @@ IDISIM_API int idisim_generate_gauss(uint64_t seed, uint64_t rng_stream, int64_t
 IDISIM_API int idisim_chargesharing(const double *counts, int64_t n0, int64_t n1, double sigma, double gain_noise,
                                     uint64_t seed, double *out, void *stream);
 
+/* Time-integrated intensity of decaying emitters (replaces cutime.simulate,
+   cutime.py:9-138 and simtime.cu:43-242, and the CPU time.simulate, time.py:47-98).
+   Emitter j is excited at times[j] and radiates a_j exp(-(t - t_j)/tau) from its
+   arrival time t_j at the pixel on; out[x, y] = integral of |E(t)|^2 dt.  Geometry,
+   near/far field and "scale" come from the handle (offsets and rotation included).
+   Arrival times are float32 on FP32-split handles (the reference's "mixed"
+   precision) and float64 on FP64 handles; amplitudes and sums are FP64.
+     emitters: n x 4 float64 (x, y, z, phi), times: n float64, both device;
+     c: speed of light, tau: decay time (same units as times, > 0);
+     out: n0 x n1 float64, device.  Synchronises nothing; uses handle scratch.   */
+IDISIM_API int idisim_time_forward(idisim_handle *h, const double *emitters, const double *times, int64_t n,
+                                   double c, double tau, double *out, void *stream);
+
 /* One streaming step of the 2-d FFT correlator (replaces the division and
    running statistics of correlator2d.add, cusimple.py:80-107): per element
    c = corr / maskcorr where maskcorr > 0.5, else 0; count == 1 starts mean = c,

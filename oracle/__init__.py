@@ -177,6 +177,10 @@ def _load():
             ctypes.POINTER(_CGeom), dp, dp, ctypes.c_int64, dp, dp, dp, dp, dp, ctypes.c_int,
         ]
         lib.oracle_adjoint.restype = None
+        lib.oracle_time_intensity.argtypes = [
+            ctypes.POINTER(_CGeom), dp, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, dp, ctypes.c_int,
+        ]
+        lib.oracle_time_intensity.restype = None
         _lib = lib
     return _lib
 
@@ -234,6 +238,43 @@ def adjoint_c(geom: Geometry, atoms, dLdI, field=None, amp=None, nthreads=0):
     lib.oracle_adjoint(ctypes.byref(g), _dptr(atoms), _dptr(amp), n, _dptr(f2), _dptr(dLdI),
                        _dptr(gpos), _dptr(gph), _dptr(gamp), int(nthreads))
     return gpos, gph, gamp
+
+
+def time_c(geom: Geometry, atoms, times, c, tau, nthreads=0):
+    """FP64 time-integrated intensity of decaying emitters (oracle/timesim.c) -> float64 (n0, n1)."""
+    lib = _load()
+    atoms = np.ascontiguousarray(atoms, dtype=np.float64)
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    out = np.empty((geom.n0, geom.n1))
+    g = _cgeom(geom)
+    lib.oracle_time_intensity(ctypes.byref(g), _dptr(atoms), _dptr(times), len(atoms), float(c), float(tau),
+                              _dptr(out), int(nthreads))
+    return out
+
+
+def time_pairsum(geom: Geometry, atoms, times, c, tau):
+    """The same intensity from the closed form tau/2 sum_{j,l} a_j conj(a_l) exp(-|t_j - t_l|/tau)
+    (the time integral of |sum_j a_j e^{-(t - t_j)/tau} H(t - t_j)|^2), with no sort; tiny sizes."""
+    atoms = np.asarray(atoms, dtype=np.float64)
+    times = np.asarray(times, dtype=np.float64)
+    out = np.empty((geom.n0, geom.n1))
+    for x in range(geom.n0):
+        for y in range(geom.n1):
+            px, py, pz = geom.pixels(rows=[x])
+            p = np.array([px[0, y], py[0, y], pz[0, y]])
+            R = np.linalg.norm(p)
+            if geom.nf:
+                d = np.linalg.norm(atoms[:, :3] - p, axis=1)
+                s = d - R + atoms[:, 2]
+                t = s / c + times
+                a = np.exp(1j * (s * geom.k + atoms[:, 3])) / (d if geom.scale else 1.0)
+            else:
+                s = atoms[:, :3] @ (p / R - np.array([0.0, 0.0, 1.0]))
+                t = -s / c + times
+                a = np.exp(1j * (atoms[:, 3] - s * geom.k)) / (R if geom.scale else 1.0)
+            w = np.exp(-np.abs(t[:, None] - t[None, :]) / tau)
+            out[x, y] = tau / 2 * np.real(np.einsum("j,l,jl->", a, np.conj(a), w))
+    return out
 
 
 def relative_l2(a, b):
