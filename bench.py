@@ -14,7 +14,8 @@ on the rows the CPU baseline computes.
 Other workloads (--workload): batched (256 shots, N_s ~ U{1000..50000} in a
 30 nm ball, 256^2 px of 100 um at 0.1 m, one persistent graph-replayed launch),
 large (1e6 emitters in three balls, amplitudes U[0.5, 1], 1024^2 px of 50 um at
-0.5 m, 16 shots), adjoint (4096 emitters, 256^2 px of 75 um at 0.1 m).
+0.5 m, 16 shots), adjoint (4096 emitters, 256^2 px of 75 um at 0.1 m), timedep (time-dependent
+simulation: 10000 emitters, 128^2 px, near field, float32 arrival keys).
 
 Prints ONE JSON line (rank 0).  --gpus must be 1: this build runs on one GPU.
 """
@@ -355,12 +356,47 @@ def bench_adjoint(args):
                        "plan": h.plan()}}
 
 
+def bench_timedep(args):
+    """Time-dependent simulation: 10000 emitters in a 20 nm ball, 128^2 px of 300 um at z = 0.1 m,
+    near field, tau = 1 fs-scale decay comparable to the arrival spread, float32 arrival keys."""
+    import torch
+
+    from idi_amd import _lib
+
+    rng = np.random.default_rng(5000)
+    n, n0 = 10000, 128
+    at = np.concatenate([ball(rng, n, 0.02), rng.uniform(0, 2 * np.pi, (n, 1))], 1)
+    times = rng.normal(0, 5e-4, n)  # fs
+    h = _lib.Handle(n0, n0, 300.0, 1e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    d = torch.from_numpy(at).cuda()
+    t = torch.from_numpy(times).cuda()
+    out = torch.empty((n0, n0), dtype=torch.float64, device="cuda")
+    pairs = n * n0 * n0
+
+    def step():
+        h.time_forward(d, t, 0.299792458, 1e-3, out)
+
+    step()
+    torch.cuda.synchronize()
+    timer = StepTimer()
+    timer.run(step, args.warmup)
+    wall, ms = timer.run(step, args.steps)
+    return {"metric": METRIC + " [time-dependent, decaying emitters]", "value": pairs * args.steps / wall,
+            "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp64+f32 keys", "data": "synthetic",
+            "config": {"workload": "10000 emitters in a 20 nm ball, excitation times N(0, 0.5 as), tau 1 as, "
+                                   "128x128 px of 300 um at z=0.1 m, near field; per pixel: FP64 arrival and "
+                                   "amplitude, segmented sort by arrival, decay scan + integral",
+                       "pairs_per_step": pairs}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single", choices=["single", "batched", "large", "adjoint"])
+    ap.add_argument("--workload", default="single", choices=["single", "batched", "large", "adjoint", "timedep"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
@@ -378,7 +414,7 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     res = {"single": bench_single, "batched": bench_batched, "large": bench_large,
-           "adjoint": bench_adjoint}[args.workload](args)
+           "adjoint": bench_adjoint, "timedep": bench_timedep}[args.workload](args)
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.rand(n, device="cuda")
