@@ -91,6 +91,9 @@ def load():
     lib.idisim_generate_ball.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl,
                                          ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     lib.idisim_chargesharing.argtypes = [vp, i64, i64, dbl, dbl, ctypes.c_uint64, vp, vp]
+    lib.idisim_generate_choice.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, i64, vp, vp,
+                                           ctypes.POINTER(ctypes.c_double), ip, vp, vp,
+                                           ctypes.POINTER(ctypes.c_size_t), vp]
     lib.idisim_time_forward.argtypes = [vp, vp, vp, i64, dbl, dbl, vp, vp]
     lib.idisim_corr_accumulate.argtypes = [vp, vp, i64, i64, vp, vp, vp]
     lib.idisim_qcorr.argtypes = [vp, vp, i64, i64, dbl, dbl, dbl, i64, i64, vp, vp]
@@ -100,7 +103,7 @@ def load():
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
                  "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
                  "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss",
-                 "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward"):
+                 "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward", "idisim_generate_choice"):
         getattr(lib, name).restype = ip
     if lib.idisim_version() != 1:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
@@ -220,6 +223,49 @@ def qcorr(zd, img, xcenter, ycenter, z, qmax, maxdqz, out):
     check(lib.idisim_qcorr(_ptr(zd), _ptr(img), int(zd.shape[0]), int(zd.shape[1]), float(xcenter), float(ycenter),
                            float(z), int(qmax), int(maxdqz), _ptr(out), _stream()))
     return out
+
+
+class ChoiceSampler:
+    """Draws (N, 4) emitters as a choice of N of the given sites without replacement, on the device.
+
+    sites: (M, 3) positions; weights: (M,) non-negative or None (uniform).  The sites,
+    weights and workspace stay on the device between draws.
+    """
+
+    def __init__(self, sites, weights=None, device=None):
+        import torch
+
+        torch_ = _require_cuda()
+        self._lib = load()
+        dev = torch_.device("cuda", torch_.cuda.current_device() if device is None else int(device))
+        self.sites = torch.as_tensor(np.ascontiguousarray(sites, dtype=np.float64).reshape(-1, 3)).to(dev)
+        self.set_weights(weights)
+        need = ctypes.c_size_t(0)
+        check(self._lib.idisim_generate_choice(0, 0, 0, len(self.sites), None, None, None, 0, None, None,
+                                               ctypes.byref(need), None))
+        self.workspace = torch.empty(int(need.value), dtype=torch.uint8, device=dev)
+
+    def set_weights(self, weights):
+        import torch
+
+        if weights is None:
+            self.weights = None
+        else:
+            w = np.ascontiguousarray(weights, dtype=np.float64).ravel()
+            if len(w) != len(self.sites):
+                raise ValueError("one weight per site")
+            self.weights = torch.as_tensor(w).to(self.sites.device)
+
+    def draw(self, out, seed, rng_stream=0, rot=None, phases=True):
+        _check_emitter_out(out)
+        r = None if rot is None else (ctypes.c_double * 9)(*[float(v) for v in np.asarray(rot, np.float64).ravel()])
+        nb = ctypes.c_size_t(self.workspace.numel())
+        check(self._lib.idisim_generate_choice(ctypes.c_uint64(int(seed) & (2**64 - 1)),
+                                               ctypes.c_uint64(int(rng_stream) & (2**64 - 1)), int(out.shape[0]),
+                                               len(self.sites), _ptr(self.sites), _ptr(self.weights), r,
+                                               int(bool(phases)), _ptr(out), _ptr(self.workspace), ctypes.byref(nb),
+                                               _stream()))
+        return out
 
 
 def chargesharing(counts, out, sigma, gain_noise, seed):

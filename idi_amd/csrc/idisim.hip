@@ -778,6 +778,32 @@ int idisim_qcorr(const float* zd, const float* img, int64_t nr, int64_t nc, doub
     return IDISIM_OK;
 }
 
+int idisim_generate_choice(uint64_t seed, uint64_t rng_stream, int64_t n, int64_t nsites, const double* sites,
+                           const double* weights, const double* rot, int phases, double* out, void* workspace,
+                           size_t* workspace_bytes, void* stream)
+{
+    g_err.clear();
+    if (!workspace_bytes) return fail(IDISIM_E_INVALID, "idisim_generate_choice: NULL workspace_bytes");
+    if (n < 0 || nsites < 1 || nsites >= ((int64_t)1 << 31))
+        return fail(IDISIM_E_INVALID, "idisim_generate_choice: need n >= 0 and 1 <= nsites < 2^31");
+    const size_t need = choice_workspace_bytes(nsites);
+    if (!workspace) {
+        *workspace_bytes = need;
+        return IDISIM_OK;
+    }
+    if (*workspace_bytes < need)
+        return fail(IDISIM_E_INVALID, "idisim_generate_choice: workspace smaller than " + std::to_string(need) +
+                                          " bytes");
+    if (n > 0 && (!sites || !out)) return fail(IDISIM_E_INVALID, "idisim_generate_choice: NULL buffer");
+    if (rot)
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(rot[k])) return fail(IDISIM_E_INVALID, "idisim_generate_choice: rot must be finite");
+    if (n == 0) return IDISIM_OK;
+    IDISIM_HIP(launch_choice(seed, rng_stream, n, nsites, sites, weights, rot, phases ? 1 : 0, out, workspace,
+                             static_cast<hipStream_t>(stream)));
+    return IDISIM_OK;
+}
+
 int idisim_chargesharing(const double* counts, int64_t n0, int64_t n1, double sigma, double gain_noise,
                          uint64_t seed, double* out, void* stream)
 {

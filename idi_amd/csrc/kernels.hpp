@@ -83,6 +83,11 @@ TimeLayout time_layout(const Geom& g, int64_t n, bool fp64_keys, size_t budget);
 hipError_t launch_time(const Geom& g, const TimeLayout& L, const double* emit, const double* times, int64_t n,
                        double c, double tau, void* scratch, double* out, hipStream_t st);
 
+// Choice of n of m sites without replacement, optionally weighted (choice.hip).
+size_t choice_workspace_bytes(int64_t m);
+hipError_t launch_choice(uint64_t seed, uint64_t stream, int64_t n, int64_t m, const double* sites, const double* w,
+                         const double* rot, int phases, double* out, void* ws, hipStream_t st);
+
 // Detector charge sharing / per-pixel gain noise (photons.hip).
 hipError_t launch_chargesharing(const double* counts, int n0, int n1, double s, double ns, uint64_t seed,
                                 double* out, hipStream_t st);

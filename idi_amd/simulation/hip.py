@@ -141,14 +141,32 @@ def _device_source(simobject):
         return "ball"
     if type(simobject) is _so.multisphere:
         return "balls"
+    if isinstance(simobject, _so.crystal):
+        return "choice"
     if isinstance(simobject, _so.gnorm) and _np.all(simobject.rho == 2):
         return "gauss"
     return None
 
 
-def _device_draw(source, simobject, out, seed):
+def _device_draw(source, simobject, out, seed, cache=None):
     """Draw the emitters of one image into `out` on the device; returns the extent for the planner
-    (0.0: let the library measure it)."""
+    (0.0: let the library measure it).  `cache` keeps device copies (crystal sites) between images."""
+    if source == "choice":
+        cache = {} if cache is None else cache
+        sites = _np.asarray(simobject._allpos, dtype=_np.float64)
+        if "sampler" not in cache:
+            cache["sampler"] = _lib.ChoiceSampler(sites, device=out.device.index)
+            cache["extent"] = float(_np.linalg.norm(sites, axis=1).max())
+            cache["weights_id"] = None
+        smp = cache["sampler"]
+        rot = simobject.orientation()
+        if simobject.fwhm is not None:
+            w = simobject.site_weights(sites if rot is None else sites @ rot.T)
+            if rot is not None or cache["weights_id"] != id(w):
+                smp.set_weights(w)
+                cache["weights_id"] = id(w)
+        smp.draw(out, seed, rot=rot)
+        return cache["extent"]
     if source == "ball":
         _lib.generate_ball(out, seed, 0, simobject.r)
         return float(simobject.r)
@@ -176,8 +194,10 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
     has been launched.
 
     device_rng=True (extension): ``simobj.sphere``, ``simobj.multisphere`` (ball
-    centres and counts still drawn on the host) and ``simobj.gauss`` / ``gnorm``
-    with rho = 2 are drawn on the GPU (idisim_generate_ball / _balls / _gauss,
+    centres and counts still drawn on the host), ``simobj.gauss`` / ``gnorm``
+    with rho = 2 and the crystals (site choice on the device, orientation and
+    weights on the host) are drawn on the GPU (idisim_generate_ball / _balls /
+    _gauss / _choice,
     Philox4x32-10, seed of image i drawn from ``simobject.rng``), and an object
     with fixed positions (rndPos False) and random phases gets its phases drawn
     there; no emitters are copied per image.
@@ -203,6 +223,7 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
             outs = [torch.empty((n0, n1), dtype=dtype, device=dev) for _ in range(2)]
 
             source = _device_source(simobject) if device_rng else None
+            draw_cache = {}
             if source == "phases":
                 fixed = simobject.get()
                 for st in stages:
@@ -221,7 +242,7 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                         _lib.generate_ball(st.d, seed, 0, positions=False)
                         ext = fixed_extent
                     else:
-                        ext = _device_draw(source, simobject, st.d, seed)
+                        ext = _device_draw(source, simobject, st.d, seed, draw_cache)
                     if amp is not None:
                         st.da.copy_(torch.from_numpy(amp))
                 h.forward(st.d, st.da if amp is not None else None, outs[slot], kind, ext)

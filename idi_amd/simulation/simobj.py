@@ -300,20 +300,31 @@ class crystal(simobj):
             atoms = atoms + rng.uniform(0, sigma, atoms.shape).astype(_np.float32)
         return atoms - (atoms.min(0) + atoms.max(0)) / 2.0
 
-    def updatePos(self):
-        pos = self._allpos
-        if _np.any(self.rndOrientation):
-            if _np.size(self.rndOrientation) == 3:
-                rot = _axisrotation(_np.array(self.rndOrientation, dtype=_np.float64), self.rng.uniform(0, 2 * pi))
-            else:
-                rot = _random_rotation(self.rng)
-            self._rotmatrix = rot @ self._rotmatrix if self._rotmatrix is not False else rot
-            pos = pos @ self._rotmatrix.T
-            self._p = None
-        if self.fwhm is not None and self._p is None:
+    def orientation(self):
+        """Rotation of the stored sites for this draw (a new random one with rndOrientation), or None."""
+        if not _np.any(self.rndOrientation):
+            return None
+        if _np.size(self.rndOrientation) == 3:
+            rot = _axisrotation(_np.array(self.rndOrientation, dtype=_np.float64), self.rng.uniform(0, 2 * pi))
+        else:
+            rot = _random_rotation(self.rng)
+        self._rotmatrix = rot @ self._rotmatrix if self._rotmatrix is not False else rot
+        self._p = None
+        return self._rotmatrix
+
+    def site_weights(self, pos):
+        """Normalised choice weights of the (rotated) sites `pos`, or None without fwhm."""
+        if self.fwhm is None:
+            return None
+        if self._p is None:
             p = _gnorm(pos, self.fwhm, self.rho, axis=1)
             self._p = p / p.sum()
-        p = self._p if self.fwhm is not None else None
+        return self._p
+
+    def updatePos(self):
+        rot = self.orientation()
+        pos = self._allpos if rot is None else self._allpos @ rot.T
+        p = self.site_weights(pos)
         missing, idx = self.N, []
         while missing:
             new = self.rng.choice(len(self._allpos), min(missing, len(self._allpos)), replace=False, p=p)

@@ -28,6 +28,7 @@
 #ifndef IDISIM_H
 #define IDISIM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 /* The library is built with hidden visibility; only these entry points are exported. */
@@ -184,6 +185,22 @@ IDISIM_API int idisim_generate_balls(uint64_t seed, uint64_t rng_stream, int64_t
      sigma: HOST array of 3 standard deviations;  rot: HOST 3 x 3 row-major or NULL */
 IDISIM_API int idisim_generate_gauss(uint64_t seed, uint64_t rng_stream, int64_t n, const double *sigma,
                                      const double *rot, int positions, int phases, double *out, void *stream);
+
+/* Emitters chosen without replacement from fixed sites (no reference counterpart; it
+   moves the per-image draw of simobj.crystal, simobj.py:479-487, onto the GPU).  Site i
+   gets the key -ln(u_i)/weights[i] (u_i uniform, Philox4x32-10 with key `seed`); the n
+   smallest keys are chosen (Efraimidis-Spirakis: the distribution of n successive draws
+   proportional to the remaining weights, as numpy's choice(replace=False, p=weights));
+   n > nsites takes further rounds with fresh keys.  Chosen sites are written to
+   out[:, 0:3] in key order, rotated by rot when given; phases = 1 also draws
+   out[:, 3] ~ U[0, 2 pi), 0 keeps it.
+     sites: nsites x 3 float64, weights: nsites float64 or NULL (uniform), both DEVICE;
+     rot: HOST 3 x 3 row-major or NULL; out: n x 4 float64, device.
+     workspace: device scratch of *workspace_bytes bytes; with workspace == NULL the
+     required size is written to *workspace_bytes and nothing else is done.        */
+IDISIM_API int idisim_generate_choice(uint64_t seed, uint64_t rng_stream, int64_t n, int64_t nsites,
+                                      const double *sites, const double *weights, const double *rot, int phases,
+                                      double *out, void *workspace, size_t *workspace_bytes, void *stream);
 
 /* Charge sharing between neighbouring detector pixels (replaces chargesharing /
    __putcharges, common.py:26-67).  Every count of `counts` (n0 x n1 float64,

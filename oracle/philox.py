@@ -98,3 +98,27 @@ def generate_gauss(n, seed, sigma, rot=None, rng_stream=0):
     r2 = _blocks(n, seed, rng_stream, 2)
     out[:, 3] = 2 * np.pi * u53(r2[:, 0], r2[:, 1])
     return out
+
+
+def generate_choice(seed, n, sites, weights=None, rot=None, rng_stream=0):
+    """(n, 4) emitters as idisim_generate_choice draws them: the n smallest keys -ln(u_i)/w_i
+    (rounds of fresh keys beyond len(sites)), sites rotated by rot, phases from block 0x200."""
+    sites = np.asarray(sites, dtype=np.float64).reshape(-1, 3)
+    m = len(sites)
+    w = np.ones(m) if weights is None else np.asarray(weights, dtype=np.float64)
+    out = np.empty((n, 4))
+    done, rnd = 0, 0
+    while done < n:
+        r = _blocks(m, seed, rng_stream, 0x100 + rnd)
+        u = 1.0 - u53(r[:, 0], r[:, 1])
+        with np.errstate(divide="ignore"):
+            keys = np.where(w > 0, -np.log(u) / np.where(w > 0, w, 1.0), np.inf)
+        take = min(n - done, m)
+        idx = np.argsort(keys, kind="stable")[:take]
+        p = sites[idx]
+        out[done:done + take, :3] = p if rot is None else p @ np.asarray(rot, dtype=np.float64).T
+        done += take
+        rnd += 1
+    r = _blocks(n, seed, rng_stream, 0x200)
+    out[:, 3] = 2 * np.pi * u53(r[:, 0], r[:, 1])
+    return out

@@ -23,7 +23,7 @@ def test_exports_every_declared_symbol(lib):
     from idi_amd import _lib
 
     names = _lib.declared_functions()
-    assert len(names) == 17
+    assert len(names) == 18
     for n in names:
         assert hasattr(lib, n), n
     assert lib.idisim_version() == 1

@@ -89,3 +89,43 @@ def test_balls_restatement_membership():
     d0 = np.linalg.norm(a[:100, :3] - centres[0], axis=1)
     d2 = np.linalg.norm(a[100:, :3] - centres[2], axis=1)
     assert d0.max() <= 15.0 and d2.max() <= 15.0
+
+
+def test_choice_restatement_inclusion_probabilities():
+    """Exponential keys give numpy's choice(replace=False, p) distribution: compare the inclusion
+    frequency of every site with numpy's own sampler (uniform and weighted)."""
+    from oracle.philox import generate_choice
+
+    m, n, trials = 12, 4, 6000
+    sites = np.stack([np.arange(m, dtype=float), np.zeros(m), np.zeros(m)], 1)
+    w = np.linspace(0.2, 2.0, m)
+    rng = np.random.default_rng(0)
+    for weights in (None, w):
+        ours = np.zeros(m)
+        ref = np.zeros(m)
+        p = None if weights is None else weights / weights.sum()
+        for t in range(trials):
+            got = generate_choice(1000 + t, n, sites, weights)
+            idx = got[:, 0].astype(int)
+            assert len(set(idx)) == n  # without replacement
+            ours[idx] += 1
+            ref[rng.choice(m, n, replace=False, p=p)] += 1
+        ours /= trials
+        ref /= trials
+        if weights is None:
+            np.testing.assert_allclose(ours, n / m, atol=0.025)
+        np.testing.assert_allclose(ours, ref, atol=0.03)
+
+
+def test_choice_restatement_rounds_and_rotation():
+    from idi_amd.util import rotation
+    from oracle.philox import generate_choice
+
+    sites = np.random.default_rng(1).normal(size=(5, 3))
+    a = generate_choice(7, 12, sites)  # 5 + 5 + 2: each round without replacement
+    for k in range(0, 10, 5):
+        assert sorted(map(tuple, a[k:k + 5, :3])) == sorted(map(tuple, sites))
+    M = rotation(0.1, 0.2, 0.3)
+    b = generate_choice(7, 12, sites, rot=M)
+    np.testing.assert_allclose(b[:, :3], a[:, :3] @ M.T, rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(b[:, 3], a[:, 3])

@@ -108,3 +108,50 @@ def test_simulate_gen_device_rng_shapes(oracle_mod, kind):
         else:
             at = generate_gauss(3000, seed, ref_obj.fwhm / (2 * np.sqrt(2 * np.log(2))))
         assert rel_l2(imgs[i], oracle_mod.field_c(g, at)) <= 1e-4
+
+
+def test_device_choice_matches_restatement():
+    """Same sites chosen in the same order as the restatement (uniform, weighted, several rounds, rotation)."""
+    import torch
+
+    from idi_amd import _lib
+    from idi_amd.util import rotation
+    from oracle.philox import generate_choice
+
+    rng = np.random.default_rng(3)
+    sites = rng.normal(size=(50_000, 3)) * 0.01
+    w = np.exp(-np.sum(sites**2, 1) / (2 * 0.008**2))
+    M = rotation(0.4, -0.2, 0.9)
+    for weights, n, rot in ((None, 30_000, None), (w, 20_000, M), (None, 120_001, None)):
+        smp = _lib.ChoiceSampler(sites, weights)
+        out = torch.empty((n, 4), dtype=torch.float64, device="cuda")
+        smp.draw(out, seed=99, rng_stream=5, rot=rot)
+        got = out.cpu().numpy()
+        ref = generate_choice(99, n, sites, weights, rot, rng_stream=5)
+        np.testing.assert_array_equal(got[:, 3], ref[:, 3])
+        np.testing.assert_allclose(got[:, :3], ref[:, :3], rtol=1e-13, atol=1e-17)
+
+
+def test_simulate_gen_device_rng_crystal(oracle_mod):
+    """A simple-cubic crystal with fresh site choice and phases per image, drawn on the device."""
+    import oracle
+    from oracle.philox import generate_choice
+
+    from idi_amd.simulation import hip, simobj
+
+    def make():
+        o = simobj.sc(8048.0, 3000, 0.0003, repeats=(20, 20, 20))
+        o.rng = np.random.default_rng(31)
+        return o
+
+    obj = make()
+    gen = hip.simulate_gen(obj, 24, 100.0, 1e5, K_UM, settings="single nf", maximg=2, device_rng=True,
+                           returngpu=True)
+    imgs = [g.cpu().numpy() for g in gen]
+    assert not np.allclose(imgs[0], imgs[1])
+    ref_obj = make()
+    g = oracle.Geometry(24, 24, 100.0, 1e5, K_UM, nf=True)
+    for i in range(2):
+        seed = int(ref_obj.rng.integers(2**63))
+        at = generate_choice(seed, 3000, ref_obj._allpos)
+        assert rel_l2(imgs[i], oracle_mod.field_c(g, at)) <= 1e-4
