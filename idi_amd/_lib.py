@@ -22,6 +22,7 @@ NF, SCALE, SECONDORDER, ROTATE = 1, 2, 4, 8
 PREC_FP64, PREC_FP32SPLIT = 0, 1
 OUT_FIELD, OUT_INTENSITY = 0, 1
 OPT_AUTOTUNE, OPT_TILE, OPT_SPLIT, OPT_SUMMODE, OPT_GRAPH, OPT_TOL_PPB, OPT_FALLBACK = 1, 2, 3, 4, 5, 6, 7
+OPT_TIME_PATH = 8
 E_PRECISION = -4
 
 

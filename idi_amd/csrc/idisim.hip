@@ -72,6 +72,7 @@ struct idisim_handle {
     Geom g{};
     // options
     int autotune = 1, force_tile = -1, force_split = 0, summode = 0, use_graph = 1, fallback = 1;
+    int time_path = 0;
     double tol_rev = 5e-6;
     // planner tables
     double du_max[kNumTiles] = {0, 0, 0, 0};
@@ -650,6 +651,10 @@ int idisim_set_option(idisim_handle* h, int option, int64_t value)
             h->tol_rev = (double)value * 1e-9;
             break;
         case IDISIM_OPT_FALLBACK: h->fallback = value ? 1 : 0; break;
+        case IDISIM_OPT_TIME_PATH:
+            if (value < 0 || value > 2) return fail(IDISIM_E_INVALID, "IDISIM_OPT_TIME_PATH must be 0, 1 or 2");
+            h->time_path = (int)value;
+            break;
         default: return fail(IDISIM_E_INVALID, "unknown option " + std::to_string(option));
     }
     h->tuned.clear();
@@ -744,11 +749,12 @@ int idisim_time_forward(idisim_handle* h, const double* emitters, const double* 
         return fail(IDISIM_E_INVALID, "idisim_time_forward: c and tau must be finite and > 0");
     DeviceGuard guard(h->device);
     const TimeLayout L = time_layout(h->g, n, h->precision == IDISIM_PREC_FP64, (size_t)1 << 30);
-    if (n > 0) {
+    if (n > 0 && !time_fused_ok(n, h->time_path)) {
         int rc = ensure_scratch(h, L.bytes);
         if (rc) return rc;
     }
-    IDISIM_HIP(launch_time(h->g, L, emitters, times, n, c, tau, h->scratch, out, static_cast<hipStream_t>(stream)));
+    IDISIM_HIP(launch_time(h->g, L, emitters, times, n, c, tau, h->time_path, h->scratch, out,
+                           static_cast<hipStream_t>(stream)));
     return IDISIM_OK;
 }
 

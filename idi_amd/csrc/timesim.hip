@@ -15,7 +15,8 @@
 // two ~detector-distance terms that the reference's formula has); arrival times are float32
 // keys on FP32-split handles (the reference's "mixed" precision) and float64 on FP64 handles.
 //
-// Work per batch of pixels:
+// Up to 10240 emitters: k_time_fused, one workgroup per pixel, all in registers and LDS.
+// Larger counts, per batch of pixels:
 //   k_time_prepare    one thread per (pixel, emitter): key t, index j, amplitude a (FP64);
 //   segmented radix sort of (t, j) per pixel (rocPRIM, stable);
 //   k_time_integrate  one workgroup per pixel: 8 sorted arrivals per lane, decayed partial
@@ -31,8 +32,41 @@
 namespace idisim {
 
 namespace {
-constexpr int kTimeK = 8;  // sorted arrivals per lane and chunk
+constexpr int kTimeK = 8;           // sorted arrivals per lane and chunk (batched path)
+constexpr int kFusedMaxItems = 40;  // arrivals per lane of the one-workgroup-per-pixel path
+constexpr int kFusedAutoMax = 10240;  // emitter counts that take that path by default
 }
+
+namespace {
+// Arrival time t and amplitude (re, im) of emitter e at pixel r (|r| = R), FP64.
+__device__ inline void arrival(const Geom& g, const double r[3], double R, const double* e, double T, double c,
+                               double* t, double* re, double* im)
+{
+    double ph, scale = 1.0;
+    if (g.nf) {
+        const double dx = e[0] - r[0], dy = e[1] - r[1], dz = e[2] - r[2];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        // d - R without cancellation: d^2 - R^2 = |r_j|^2 - 2 r_j . r_p
+        const double e2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+        const double er = e[0] * r[0] + e[1] * r[1] + e[2] * r[2];
+        const double s = (e2 - 2.0 * er) / (d + R) + e[2];
+        *t = s / c + T;
+        ph = s * g.k + e[3];
+        if (g.scale) scale = 1.0 / d;
+    } else {
+        const double rd = 1.0 / R;
+        const double kr = g.k * rd;
+        const double s = e[0] * (r[0] * kr) + e[1] * (r[1] * kr) + e[2] * (g.k * (r[2] * rd - 1.0));
+        *t = -s / (c * g.k) + T;
+        ph = e[3] - s;
+        if (g.scale) scale = rd;
+    }
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    *re = cs * scale;
+    *im = sn * scale;
+}
+}  // namespace
 
 template <typename Tt>
 __global__ __launch_bounds__(256) void k_time_prepare(Geom g, const double* __restrict__ emit,
@@ -46,33 +80,13 @@ __global__ __launch_bounds__(256) void k_time_prepare(Geom g, const double* __re
     const int64_t p = p0 + pl;
     double r[3];
     pixel_pos(g, (double)(p / g.n1), (double)(p % g.n1), r);
-    const double* e = emit + 4 * j;
-    double t, ph, scale = 1.0;
-    if (g.nf) {
-        const double dx = e[0] - r[0], dy = e[1] - r[1], dz = e[2] - r[2];
-        const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        const double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        // d - R without cancellation: d^2 - R^2 = |r_j|^2 - 2 r_j . r_p
-        const double e2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
-        const double er = e[0] * r[0] + e[1] * r[1] + e[2] * r[2];
-        const double s = (e2 - 2.0 * er) / (d + R) + e[2];
-        t = s / c + times[j];
-        ph = s * g.k + e[3];
-        if (g.scale) scale = 1.0 / d;
-    } else {
-        const double rd = 1.0 / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        const double kr = g.k * rd;
-        const double s = e[0] * (r[0] * kr) + e[1] * (r[1] * kr) + e[2] * (g.k * (r[2] * rd - 1.0));
-        t = -s / (c * g.k) + times[j];
-        ph = e[3] - s;
-        if (g.scale) scale = rd;
-    }
-    double sn, cs;
-    sincos(ph, &sn, &cs);
+    const double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double t, re, im;
+    arrival(g, r, R, emit + 4 * j, times[j], c, &t, &re, &im);
     const size_t o = (size_t)pl * n + j;
     keys[o] = (Tt)t;
     idx[o] = (int)j;
-    amp[o] = make_double2(cs * scale, sn * scale);
+    amp[o] = make_double2(re, im);
 }
 
 __global__ void k_time_offsets(int* __restrict__ offs, int np, int n)
@@ -191,6 +205,126 @@ __global__ __launch_bounds__(256) void k_time_integrate(const Tt* __restrict__ k
     if (tid == 0) out[p0 + pl] = ssum[0] * (0.5 * tau);
 }
 
+// Small emitter counts (n <= 256 ITEMS): one workgroup per pixel does everything in LDS.
+// The arrival keys are formed in a plain loop into LDS, moved into registers (blocked: lane l
+// holds emitters l ITEMS .. l ITEMS + ITEMS - 1) for rocPRIM's block radix sort (stable) and
+// written back sorted, with the emitter indices, to a padded LDS array (odd stride: no bank
+// conflicts).  The amplitudes are recomputed in sorted order: once for the lanes' aggregate
+// states (scanned across lanes as above) and once more to step the field through the lane's
+// arrivals from its incoming state and sum the integral terms.  The key buffer, the sort's
+// storage and the sorted arrays share one LDS union; nothing goes through global memory.
+template <typename Tt, int ITEMS>
+__global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __restrict__ emit,
+                                                    const double* __restrict__ times, int64_t n, double c,
+                                                    double tau, double* __restrict__ out)
+{
+    using Sort = rocprim::block_radix_sort<Tt, 256, ITEMS, int>;
+    constexpr int S = ITEMS + 1;  // padded stride of the sorted arrays
+    union Lds {
+        typename Sort::storage_type sort;
+        Tt kbuf[256 * ITEMS];
+        struct {
+            Tt key[256 * S];
+            int idx[256 * S];
+        } sorted;
+    };
+    __shared__ Lds u;
+    __shared__ double sre[256], sim[256], st[256];
+    __shared__ int shas[256];
+    __shared__ double ssum[256];
+    const int tid = threadIdx.x;
+    const int64_t p = blockIdx.x;
+    const double inv_tau = 1.0 / tau, two_over_tau = 2.0 / tau;
+    double r[3];
+    pixel_pos(g, (double)(p / g.n1), (double)(p % g.n1), r);
+    const double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+
+    // arrival keys (emitters in order, coalesced reads)
+#pragma unroll 1
+    for (int j = tid; j < 256 * ITEMS; j += 256) {
+        Tt key = (Tt)INFINITY;
+        if (j < n) {
+            double t, re, im;
+            arrival(g, r, R, emit + 4 * (int64_t)j, times[j], c, &t, &re, &im);
+            key = (Tt)t;
+        }
+        u.kbuf[j] = key;
+    }
+    __syncthreads();
+    Tt keys[ITEMS];
+    int js[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        keys[k] = u.kbuf[tid * ITEMS + k];
+        js[k] = tid * ITEMS + k;
+    }
+    __syncthreads();  // kbuf read by every lane before the sort reuses the storage
+    Sort().sort(keys, js, u.sort);
+    __syncthreads();  // sort storage consumed before it becomes the sorted arrays
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        u.sorted.key[tid * S + k] = keys[k];
+        u.sorted.idx[tid * S + k] = js[k];
+    }
+    const Tt* skey = u.sorted.key + tid * S;
+    const int* sidx = u.sorted.idx + tid * S;
+    const int64_t i0 = (int64_t)tid * ITEMS;
+    const int cnt = (int)((n - i0) < ITEMS ? ((n - i0) > 0 ? n - i0 : 0) : ITEMS);
+
+    // lane aggregate over this lane's sorted arrivals (its own LDS entries)
+    DecayState mine = {0.0, 0.0, 0.0, 0};
+#pragma unroll 1
+    for (int k = 0; k < cnt; ++k) {
+        const int j = sidx[k];
+        double t, re, im;
+        arrival(g, r, R, emit + 4 * (int64_t)j, times[j], c, &t, &re, &im);
+        mine = combine(mine, {re, im, (double)skey[k], 1}, inv_tau);
+    }
+    sre[tid] = mine.re;
+    sim[tid] = mine.im;
+    st[tid] = mine.t;
+    shas[tid] = mine.has;
+    __syncthreads();  // also publishes every lane's sorted entries
+    DecayState inc = mine;
+    for (int off = 1; off < 256; off <<= 1) {
+        DecayState left = {0.0, 0.0, 0.0, 0};
+        if (tid >= off) left = {sre[tid - off], sim[tid - off], st[tid - off], shas[tid - off]};
+        __syncthreads();
+        inc = combine(left, inc, inv_tau);
+        sre[tid] = inc.re;
+        sim[tid] = inc.im;
+        st[tid] = inc.t;
+        shas[tid] = inc.has;
+        __syncthreads();
+    }
+    DecayState x = {0.0, 0.0, 0.0, 0};
+    if (tid > 0) x = {sre[tid - 1], sim[tid - 1], st[tid - 1], shas[tid - 1]};
+
+    double acc = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < cnt; ++k) {
+        const int j = sidx[k];
+        double t, re, im;
+        arrival(g, r, R, emit + 4 * (int64_t)j, times[j], c, &t, &re, &im);
+        const double tk = (double)skey[k];
+        x = combine(x, {re, im, tk, 1}, inv_tau);
+        const double m2 = x.re * x.re + x.im * x.im;
+        if (i0 + k + 1 < n) {
+            const double tn = (k + 1 < ITEMS) ? (double)skey[k + 1] : (double)u.sorted.key[(tid + 1) * S];
+            acc += -m2 * expm1((tk - tn) * two_over_tau);
+        } else {
+            acc += m2;
+        }
+    }
+    ssum[tid] = acc;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (tid < s2) ssum[tid] += ssum[tid + s2];
+        __syncthreads();
+    }
+    if (tid == 0) out[p] = ssum[0] * (0.5 * tau);
+}
+
 template <typename Tt>
 static size_t time_sort_bytes(int np, int64_t n)
 {
@@ -256,19 +390,53 @@ static hipError_t run_time(const Geom& g, const TimeLayout& L, const double* emi
     return hipSuccess;
 }
 
+template <typename Tt>
+static hipError_t run_fused(const Geom& g, const double* emit, const double* times, int64_t n, double c, double tau,
+                            double* out, int64_t npix, hipStream_t st)
+{
+    const dim3 grid((unsigned)npix);
+#define IDISIM_TIME_FUSED(I) \
+    hipLaunchKernelGGL((k_time_fused<Tt, I>), grid, dim3(256), 0, st, g, emit, times, n, c, tau, out)
+    if (n <= 256 * 4)
+        IDISIM_TIME_FUSED(4);
+    else if (n <= 256 * 8)
+        IDISIM_TIME_FUSED(8);
+    else if (n <= 256 * 16)
+        IDISIM_TIME_FUSED(16);
+    else if (n <= 256 * 24)
+        IDISIM_TIME_FUSED(24);
+    else if (n <= 256 * 32)
+        IDISIM_TIME_FUSED(32);
+    else
+        IDISIM_TIME_FUSED(kFusedMaxItems);
+#undef IDISIM_TIME_FUSED
+    return hipGetLastError();
+}
+
 __global__ void k_time_zero(double* __restrict__ out, int64_t npix)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < npix) out[i] = 0.0;
 }
 
+bool time_fused_ok(int64_t n, int path)
+{
+    if (path == 2 || n > 256 * kFusedMaxItems) return false;
+    return path == 1 || n <= kFusedAutoMax;
+}
+
 hipError_t launch_time(const Geom& g, const TimeLayout& L, const double* emit, const double* times, int64_t n,
-                       double c, double tau, void* scratch, double* out, hipStream_t st)
+                       double c, double tau, int path, void* scratch, double* out, hipStream_t st)
 {
     if (n == 0) {
         const int64_t npix = (int64_t)g.n0 * g.n1;
         hipLaunchKernelGGL(k_time_zero, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, out, npix);
         return hipGetLastError();
+    }
+    if (time_fused_ok(n, path)) {
+        const int64_t npix = (int64_t)g.n0 * g.n1;
+        if (L.fp64) return run_fused<double>(g, emit, times, n, c, tau, out, npix, st);
+        return run_fused<float>(g, emit, times, n, c, tau, out, npix, st);
     }
     return L.fp64 ? run_time<double>(g, L, emit, times, n, c, tau, scratch, out, st)
                   : run_time<float>(g, L, emit, times, n, c, tau, scratch, out, st);

@@ -78,6 +78,9 @@ extern "C" {
                                      revolution (default 5000 = 5e-6 rev)          */
 #define IDISIM_OPT_FALLBACK     7 /* 1 (default): FP32-split handles run the FP64
                                      kernel when no tile meets the bound; 0: error */
+#define IDISIM_OPT_TIME_PATH    8 /* time-dependent simulation: 0 (default) picks
+                                     by emitter count, 1 = one workgroup per pixel
+                                     (n <= 10240), 2 = batched sort path        */
 
 typedef struct idisim_handle idisim_handle;
 
