@@ -100,3 +100,20 @@ def test_cucor_reference_case():
         assert np.unravel_index(np.argmax(c), c.shape) == tuple(i // 2 for i in c.shape)
     with pytest.raises(ValueError):
         f(t)
+
+
+def test_qcorr_wide_image_unstaged_rows():
+    """Images wider than 1024 columns read their rows from global memory instead of LDS."""
+    import torch
+
+    from idi_amd import _lib
+    from idi_amd.reconstruction.qcor import geometry
+    from oracle.correlate import qcorr
+
+    rng = np.random.default_rng(12)
+    img = rng.uniform(0, 1, (3, 1100)).astype(np.float32)
+    zd, xc, yc, qmax, maxdqz = geometry(img.shape, 500.0, 32)
+    out = torch.empty((2 * maxdqz, 2 * qmax, 2 * qmax), dtype=torch.float64, device="cuda")
+    _lib.qcorr(torch.from_numpy(zd).cuda(), torch.from_numpy(img).cuda(), xc, yc, 500.0, qmax, maxdqz, out)
+    np.testing.assert_allclose(out.cpu().numpy(), qcorr(zd, img, xc, yc, 500.0, qmax, maxdqz), rtol=1e-12,
+                               atol=1e-11)

@@ -155,3 +155,17 @@ def test_simulate_gen_device_rng_crystal(oracle_mod):
         seed = int(ref_obj.rng.integers(2**63))
         at = generate_choice(seed, 3000, ref_obj._allpos)
         assert rel_l2(imgs[i], oracle_mod.field_c(g, at)) <= 1e-4
+
+
+def test_device_choice_skips_zero_weights():
+    import torch
+
+    from idi_amd import _lib
+
+    sites = np.stack([np.arange(100.0), np.zeros(100), np.zeros(100)], 1)
+    w = np.where(np.arange(100) % 2 == 0, 1.0, 0.0)
+    smp = _lib.ChoiceSampler(sites, w)
+    out = torch.empty((50, 4), dtype=torch.float64, device="cuda")
+    smp.draw(out, seed=4)
+    chosen = out.cpu().numpy()[:, 0].astype(int)
+    assert sorted(chosen) == list(range(0, 100, 2))  # all 50 positive-weight sites, none of the others

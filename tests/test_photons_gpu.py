@@ -79,3 +79,14 @@ def test_dropin_chargesharing_after_randomphotons():
     assert common.chargesharing(photons_img, 0.0) is photons_img
     with pytest.raises(ValueError):
         common.chargesharing(-photons_img - 1, 0.5, rng=1)
+
+
+def test_chargesharing_empty_image():
+    import torch
+
+    from idi_amd import _lib
+
+    d = torch.zeros((13, 9), dtype=torch.float64, device="cuda")
+    out = torch.full_like(d, 7.0)
+    _lib.chargesharing(d, out, 1.2, 0.1, seed=3)
+    assert float(out.abs().max()) == 0.0

@@ -103,3 +103,29 @@ def test_dropin_time_modules(lib, oracle_mod):
     assert rel_l2(c, oracle_mod.time_c(gff, at, times, C_UM_FS, 1e-3)) <= 1e-10
     imgs = list(cutime.simulate_gen(obj, 8, 300.0, 1e5, K_UM, C_UM_FS, 1e-3, 1e-3, maximg=2))
     assert len(imgs) == 2 and imgs[0].shape == (8, 8) and not np.allclose(imgs[0], imgs[1])
+
+
+def test_time_paths_agree_and_empty_input(lib):
+    """The one-workgroup-per-pixel path and the batched sort path give the same image; no emitters
+    give a zero image."""
+    import torch
+
+    import oracle
+    from gpu_util import make_handle
+
+    rng = np.random.default_rng(21)
+    at = _atoms(rng, 3000, 0.02)
+    times = rng.normal(0, 5e-4, 3000)
+    g = oracle.Geometry(20, 17, 300.0, 1e5, K_UM, nf=True)
+    h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+    d, t = torch.from_numpy(at).cuda(), torch.from_numpy(times).cuda()
+    res = []
+    for path in (1, 2):
+        h.set_option(lib.OPT_TIME_PATH, path)
+        res.append(h.time_forward(d, t, C_UM_FS, 1e-3).cpu().numpy())
+    assert rel_l2(res[0], res[1]) <= 1e-12
+    h.set_option(lib.OPT_TIME_PATH, 0)
+    empty = h.time_forward(d[:0], t[:0], C_UM_FS, 1e-3).cpu().numpy()
+    assert np.all(empty == 0)
+    with pytest.raises(lib.IdisimError):
+        h.time_forward(d, t, C_UM_FS, -1.0)
