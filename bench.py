@@ -11,7 +11,8 @@ literal FP64 kernel is timed the same way as the comparator.  The FP32-split
 image is checked against the FP64 GPU image (full) and against the FP64 oracle
 on the rows the CPU baseline computes.
 
-Other workloads (--workload): batched (256 shots, N_s ~ U{1000..50000} in a
+Other workloads (--workload): c1 (1000 emitters, 128^2 px, the plain FP64 kernel and
+the CPU restatement on the full image), batched (256 shots, N_s ~ U{1000..50000} in a
 30 nm ball, 256^2 px of 100 um at 0.1 m, one persistent graph-replayed launch),
 large (1e6 emitters in three balls, amplitudes U[0.5, 1], 1024^2 px of 50 um at
 0.5 m, 16 shots), adjoint (4096 emitters, 256^2 px of 75 um at 0.1 m), timedep (time-dependent
@@ -247,6 +248,22 @@ def bench_batched(args):
     timer = StepTimer()
     timer.run(step, args.warmup)
     wall, ms = timer.run(step, args.steps)
+    # the plain FP64 kernel on the same batch (the emitters of the last step): throughput and the
+    # relative L2 error of the FP32-split intensities over all 256 shots
+    hf = _lib.Handle(n0, n1, 100.0, 1e5, K, nf=True, precision=_lib.PREC_FP64)
+    outf = torch.empty((nshots, n0, n1), dtype=torch.float64, device="cuda")
+
+    def step64():
+        hf.forward_batched(d, offsets, None, outf, _lib.OUT_INTENSITY, ext)
+
+    step64()
+    torch.cuda.synchronize()
+    f64_steps = max(1, args.steps // 3)
+    wall64, _ = StepTimer().run(step64, f64_steps)
+    rel = float((out.double() - outf).norm() / outf.norm())
+    f64 = {"value": pairs * f64_steps / wall64, "ms_per_step": wall64 * 1e3 / f64_steps, "plan": hf.plan()}
+    print(f"fp64 batched: {f64['ms_per_step']:.1f} ms/step; relL2(I) fp32-split vs fp64 over 256 shots {rel:.3e}",
+          file=sys.stderr)
     return {"metric": METRIC + " [batched 256 shots]", "value": pairs * args.steps / wall, "unit": "pair-evals/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -254,6 +271,8 @@ def bench_batched(args):
                                    "phases per step drawn on the GPU (Philox), 256x256 px of 100 um at z=0.1 m, "
                                    "near field, intensity out", "pairs_per_step": pairs,
                        "plan": h.plan()},
+            "fp64": f64, "speedup_vs_fp64": (pairs * args.steps / wall) / f64["value"],
+            "rel_l2_intensity": {"fp32split_vs_fp64_gpu_all_shots": rel},
             "roofline": {"bound": "valu", "achieved": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
@@ -356,6 +375,49 @@ def bench_adjoint(args):
                        "plan": h.plan()}}
 
 
+def bench_c1(args):
+    """The reference's CPU-runnable case: 1000 emitters in a 50 nm ball, 128^2 px of 75 um at 0.1 m,
+    near field.  Plain FP64 kernel and FP32-split on the GPU, the FP64 CPU restatement on the full image."""
+    import torch
+
+    import oracle
+    from idi_amd import _lib
+
+    rng = np.random.default_rng(1000)
+    n, n0 = 1000, 128
+    atoms = np.concatenate([ball(rng, n, 0.05), rng.uniform(0, 2 * np.pi, (n, 1))], 1)
+    geom = oracle.Geometry(n0, n0, 75.0, 1e5, K, nf=True)
+    pairs = n * n0 * n0
+    d = torch.from_numpy(atoms).cuda()
+    ext = float(np.linalg.norm(atoms[:, :3], axis=1).max())
+    res = {}
+    for name, prec in (("fp64", _lib.PREC_FP64), ("fp32split", _lib.PREC_FP32SPLIT)):
+        h = _lib.Handle(n0, n0, 75.0, 1e5, K, nf=True, precision=prec)
+        out = h.forward(d, None, None, _lib.OUT_FIELD, ext)
+        torch.cuda.synchronize()
+        timer = StepTimer()
+        timer.run(lambda: h.forward(d, None, out, _lib.OUT_FIELD, ext), args.warmup)
+        wall, _ = timer.run(lambda: h.forward(d, None, out, _lib.OUT_FIELD, ext), args.steps)
+        res[name] = {"value": pairs * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps, "plan": h.plan(),
+                     "field": out.cpu().numpy()}
+    nthreads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    ref = oracle.field_c(geom, atoms, nthreads=nthreads)
+    cpu_s = time.perf_counter() - t0
+    I_ref = np.abs(ref) ** 2
+    rel = {k: rel_l2(np.abs(v.pop("field")) ** 2, I_ref) for k, v in res.items()}
+    return {"metric": METRIC + " [C1: plain FP64, 1000 emitters, 128^2]", "value": res["fp64"]["value"],
+            "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": res["fp64"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+            "config": {"workload": "1000 emitters uniform in a 50 nm ball, random phases, 128x128 px of 75 um at "
+                                   "z=0.1 m, near field, plain FP64 kernel", "pairs_per_step": pairs,
+                       "plan": res["fp64"]["plan"]},
+            "fp32split": res["fp32split"], "rel_l2_intensity_vs_oracle": rel,
+            "cpu_baseline": {"value": pairs / cpu_s, "unit": "pair-evals/s", "cores": nthreads, "kind": "port",
+                             "sample": "the full 128x128 image; FP64 C/OpenMP restatement of cpu.py:30-107"}}
+
+
 def bench_timedep(args):
     """Time-dependent simulation: 10000 emitters in a 20 nm ball, 128^2 px of 300 um at z = 0.1 m,
     near field, tau = 1 fs-scale decay comparable to the arrival spread, float32 arrival keys."""
@@ -396,7 +458,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single", choices=["single", "batched", "large", "adjoint", "timedep"])
+    ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
@@ -414,7 +476,7 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     res = {"single": bench_single, "batched": bench_batched, "large": bench_large,
-           "adjoint": bench_adjoint, "timedep": bench_timedep}[args.workload](args)
+           "adjoint": bench_adjoint, "timedep": bench_timedep, "c1": bench_c1}[args.workload](args)
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.rand(n, device="cuda")
