@@ -13,6 +13,10 @@
 // so per (tile, emitter) only S = sum_p Im(w_p e^{i theta}) and V = sum_p Im(.) dq_p
 // (3 floats) are needed in FP32 (plus sum Re(.) for dL/da); they are folded into FP64
 // accumulators with the FP64 tile-centre vector u_c after every tile.
+// With the 1/r "scale" model the amplitude is R_c/(D_cj R_p): 1/R_p goes into w'_p and R_c/D_cj
+// multiplies the tile's sums when they are folded.  The gradient of the amplitude itself,
+// Re(.) grad(1/D), is left out: next to the phase term it is smaller by 1/(k D) (about 1e-9
+// for X-rays at a detector distance of 0.1 m).
 // With w'_p = m_p e^{2 pi i psi_p}, Im(w'_p e^{2 pi i tau}) = m_p sin(2 pi (tau + psi_p)):
 // one transcendental per pair, and m_p, m_p dq_p are per-pixel table entries, so
 // S and V are four FMAs on that sine (the cosine is needed for dL/da only).
@@ -73,6 +77,10 @@ __global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* _
             const double r2 = wr * cs - wi * sn;
             wi = wr * sn + wi * cs;
             wr = r2;
+        }
+        if (g.scale) {  // the per-pixel factor 1/R_p of the "scale" amplitude R_c/(D_cj R_p)
+            wr /= f.R;
+            wi /= f.R;
         }
         const double m = sqrt(wr * wr + wi * wi);
         const double psi = (m > 0.0) ? atan2(wi, wr) * kInv2Pi : 0.0;
@@ -135,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
         if (!active) continue;
 
         const Frame c = adj_tile_centre(g, t, tiles_y);
-        float rx[EPL], ry[EPL], rz[EPL], rA[EPL], rW[EPL], bx[EPL], by[EPL], bz[EPL];
+        float rx[EPL], ry[EPL], rz[EPL], rA[EPL], rW[EPL], bx[EPL], by[EPL], bz[EPL], rm[EPL];
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
             const int64_t e = ebase + lane + 64 * i;
@@ -147,13 +155,14 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
                 rz[i] = r.z;
                 rA[i] = r.A;
                 rW[i] = r.W;
+                rm[i] = r.amp;  // R_c / D_cj with "scale", else 1
                 // beta_cj in FP32 (|beta| ~ |r_j|/R; only its relative precision matters)
                 const float fx = (float)a4[0], fy = (float)a4[1], fz = (float)a4[2];
                 bx[i] = fx * r.iD - (float)c.rh[0] * r.sigma;
                 by[i] = fy * r.iD - (float)c.rh[1] * r.sigma;
                 bz[i] = fz * r.iD - (float)c.rh[2] * r.sigma;
             } else {
-                rx[i] = ry[i] = rz[i] = rA[i] = rW[i] = bx[i] = by[i] = bz[i] = 0.f;
+                rx[i] = ry[i] = rz[i] = rA[i] = rW[i] = bx[i] = by[i] = bz[i] = rm[i] = 0.f;
             }
         }
 
@@ -191,18 +200,20 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
         const double ik = 1.0 / g.krev;
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
-            const double S = (double)Sim[i];
-            acc[0][i] += g.k * (((double)bx[i] - c.u[0]) * S + (double)Vx[i] * ik);
-            acc[1][i] += g.k * (((double)by[i] - c.u[1]) * S + (double)Vy[i] * ik);
-            acc[2][i] += g.k * (((double)bz[i] - c.u[2]) * S + (double)Vz[i] * ik);
+            const double ma = (double)rm[i];
+            const double S = ma * (double)Sim[i];
+            const double Vfx = ma * (double)Vx[i], Vfy = ma * (double)Vy[i], Vfz = ma * (double)Vz[i];
+            acc[0][i] += g.k * (((double)bx[i] - c.u[0]) * S + Vfx * ik);
+            acc[1][i] += g.k * (((double)by[i] - c.u[1]) * S + Vfy * ik);
+            acc[2][i] += g.k * (((double)bz[i] - c.u[2]) * S + Vfz * ik);
             if (!g.nf && g.secondorder) {
                 // theta += k/(2 R_p)(x^2 + y^2) with R_c/R_p = lam (lam ~ 1 when W is folded)
-                const double L = (LAM ? (double)Lm[i] : S) * g.k / c.R;
+                const double L = (LAM ? ma * (double)Lm[i] : S) * g.k / c.R;
                 acc[0][i] += L * (double)rx[i];
                 acc[1][i] += L * (double)ry[i];
             }
             acc[3][i] += S;
-            if (AMP) acc[4][i] += (double)Sre[i];
+            if (AMP) acc[4][i] += ma * (double)Sre[i];
         }
     }
 

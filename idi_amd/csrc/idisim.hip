@@ -1022,7 +1022,6 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
         return fail(IDISIM_E_INVALID, "idisim_adjoint: bad arguments");
     if (h->precision != IDISIM_PREC_FP32SPLIT)
         return fail(IDISIM_E_UNSUPPORTED, "idisim_adjoint: only FP32-split handles have an adjoint");
-    if (h->g.scale) return fail(IDISIM_E_UNSUPPORTED, "idisim_adjoint: the 1/r 'scale' model has no adjoint");
     if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_adjoint: extent must be finite");
     if (n == 0) return IDISIM_OK;
     DeviceGuard guard(h->device);

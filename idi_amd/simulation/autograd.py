@@ -6,8 +6,7 @@
 pos (N, 3) and phase (N,) (and optionally amp (N,)) are CUDA tensors; the
 returned intensity is float32 (N0, N1).  The backward pass calls
 idisim_adjoint, which computes dL/dpos, dL/dphase and dL/damp from dL/dI and
-the saved complex64 field (no reference counterpart).  The 1/r "scale" model
-has no adjoint.
+the saved complex64 field (no reference counterpart).
 """
 from __future__ import annotations
 
@@ -46,8 +45,6 @@ def intensity(pos, phase, amp=None, *, Ndet, pixelsize, detz, k, settings="singl
     """|E|^2 on the detector, differentiable w.r.t. pos, phase and amp (FP32-split)."""
     if "single" not in settings:
         settings = settings + " single"
-    if "scale" in settings:
-        raise NotImplementedError("the 1/r 'scale' model has no adjoint")
     h = get_handle(Ndet, pixelsize, detz, k, settings, detangles, detoffset, device=pos.device.index)
     if extent is None:
         extent = float(torch.linalg.vector_norm(pos.detach().to(torch.float64), dim=1).max()) if len(pos) else 0.0

@@ -147,7 +147,9 @@ IDISIM_API int idisim_forward_batched(idisim_handle *h, const double *emitters, 
 
 /* Adjoint of I = |E|^2 for a loss L(I) (no reference counterpart).
      field:      the complex64 field that idisim_forward returned for the same
-                 emitters (FP32SPLIT handles only; not with IDISIM_SCALE)
+                 emitters (FP32SPLIT handles only).  With IDISIM_SCALE the
+                 gradient of the 1/r amplitude itself is left out (smaller than
+                 the phase term by 1/(k r), ~1e-9 for X-rays)
      dLdI:       float32 n0 x n1
      grad_pos:   float32 n x 3  dL/d(x, y, z)
      grad_phase: float32 n      dL/dphi

@@ -11,7 +11,7 @@ from gpu_util import ball, make_handle, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-GRAD_RTOL = 2e-3
+GRAD_RTOL = 2e-5  # measured: 2e-7 .. 2.1e-6 (FP32 per-tile sums folded into FP64)
 
 
 @pytest.fixture(scope="module")
@@ -47,12 +47,13 @@ def _gpu_grad(lib, geom, at, target, amp=None):
     return gpos.cpu().numpy().astype(np.float64), gph.cpu().numpy().astype(np.float64), gamp.cpu().numpy(), h
 
 
-@pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder"])
+@pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder", "nf_scale", "ff_scale"])
 def test_adjoint_vs_fp64_analytic(lib, oracle_mod, mode):
     from oracle import Geometry
 
     rng = np.random.default_rng(91)
-    g = Geometry(64, 48, 300.0, 1e5, K_UM, nf=mode == "nf", secondorder="secondorder" in mode)
+    g = Geometry(64, 48, 300.0, 1e5, K_UM, nf=mode.startswith("nf"), secondorder="secondorder" in mode,
+                 scale="scale" in mode)
     at, target = _problem(rng, 600, 0.02, g, oracle_mod)
     amp = rng.uniform(0.5, 1, 600)
     field = oracle_mod.field_c(g, at, amp)
