@@ -1,7 +1,8 @@
 """FP64 oracle for idi's direct-sum wavefield (TEST INFRASTRUCTURE ONLY).
 
-Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only to CHECK the HIP library.  The
+Only ``tests/``, the smoke check of the repository's entry module and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only to CHECK the
+HIP library.  The
 product package ``idi_amd`` never imports it and has no CPU fallback.
 
 Contents
