@@ -64,6 +64,8 @@ def chargesharing(img, s, ns=0, rng=None):
     src = t.to(device=dev, dtype=torch.float64).contiguous()
     if bool((src < 0).any()):
         raise ValueError("photon counts must be non-negative")
+    if s > 0 and float(src.sum()) >= 2.0**31:
+        raise ValueError("chargesharing expects photon counts (at most 2^31 hits per image); got a larger total")
     out = torch.empty_like(src)
     _lib.chargesharing(src, out, s, ns, _seed(rng))
     return out if is_t else out.cpu().numpy()

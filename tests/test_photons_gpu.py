@@ -79,6 +79,8 @@ def test_dropin_chargesharing_after_randomphotons():
     assert common.chargesharing(photons_img, 0.0) is photons_img
     with pytest.raises(ValueError):
         common.chargesharing(-photons_img - 1, 0.5, rng=1)
+    with pytest.raises(ValueError):
+        common.chargesharing(photons_img * 1e9, 0.5, rng=1)  # intensities, not counts
 
 
 def test_chargesharing_empty_image():
