@@ -521,9 +521,14 @@ int run_batched(idisim_handle* h, const BatchCall& c, int tile, int bpc)
         h->plan.graph = 0;
         return launch_batched(h, c, tile, bpc, c.st);
     }
-    std::vector<uintptr_t> key = {(uintptr_t)c.emit, (uintptr_t)c.amp, (uintptr_t)c.out, (uintptr_t)c.kind,
-                                  (uintptr_t)h->d_items, (uintptr_t)h->nitems, (uintptr_t)tile, (uintptr_t)bpc,
-                                  (uintptr_t)h->summode, (uintptr_t)c.split_path};
+    // the W-fold decision depends on the emitter extent of this call: a graph captured with
+    // another decision must not be replayed
+    const bool folded = c.split_path && (h->g.nf || h->g.secondorder) &&
+                        fold_error(h, tile, h->rho) <= 0.1 * h->tol_rev;
+    std::vector<uintptr_t> key = {(uintptr_t)c.emit,    (uintptr_t)c.amp,       (uintptr_t)c.out,
+                                  (uintptr_t)c.kind,    (uintptr_t)h->d_items,  (uintptr_t)h->nitems,
+                                  (uintptr_t)tile,      (uintptr_t)bpc,         (uintptr_t)h->summode,
+                                  (uintptr_t)c.split_path, (uintptr_t)folded};
     if (key != h->gkey || !h->gexec) {
         if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
         if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -551,7 +556,7 @@ int run_batched(idisim_handle* h, const BatchCall& c, int tile, int bpc)
 
 int fill_zero(void* out, size_t bytes, hipStream_t st)
 {
-    if (bytes) IDISIM_HIP(hipMemsetAsync(out, 0, bytes, st));
+    IDISIM_HIP(launch_zero(out, bytes, st));
     return IDISIM_OK;
 }
 

@@ -406,6 +406,13 @@ __global__ void k_reset_counter(int* counter) { *counter = 0; }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// Zero `words` 32-bit words (a kernel, so that a captured graph holds no memset node).
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ p, int64_t words)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 // max_j |r_j|^2 as the bit pattern of a non-negative double (ordered like u64)
 __global__ __launch_bounds__(kBlock) void k_extent(const double* __restrict__ emit, int64_t n,
                                                    unsigned long long* out)
@@ -622,6 +629,17 @@ hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out,
     if (blocks > 1024) blocks = 1024;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_extent, dim3((unsigned)blocks), dim3(kBlock), 0, st, emit, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(void* out, size_t bytes, hipStream_t st)
+{
+    // every output size here is a multiple of 4 bytes (float / double / complex elements)
+    const int64_t words = (int64_t)(bytes / 4);
+    if (words == 0) return hipSuccess;
+    int64_t blocks = (words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<uint32_t*>(out), words);
     return hipGetLastError();
 }
 

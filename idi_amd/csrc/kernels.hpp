@@ -51,6 +51,7 @@ hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& 
                           float* grad_pos, float* grad_phase, float* grad_amp, hipStream_t st);
 
 hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out, hipStream_t st);
+hipError_t launch_zero(void* out, size_t bytes, hipStream_t st);  // bytes: a multiple of 4
 
 // Emitters in a ball / random phases from Philox4x32-10 (generate.hip).
 hipError_t launch_gen_ball(uint64_t seed, uint64_t stream, int64_t n, double radius, const double* centre,
