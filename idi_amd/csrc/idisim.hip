@@ -48,13 +48,22 @@ int fail(int code, const std::string& msg)
 // 1.29e-7 (2.05e-8 rev); with margin.
 constexpr double kSinCosErrRev = 3.0e-8;
 
+// A tuned choice is keyed by the call's shape and kernel family (FP32-split or
+// FP64); it is reused only while its tile is still among the eligible ones (a
+// larger emitter extent can push a tile's error bound past the tolerance).
 struct TuneKey {
-    int kind, has_amp, batched, nbucket;
+    int kind, has_amp, batched, split_path, nbucket;
     bool operator<(const TuneKey& o) const
     {
-        return std::tie(kind, has_amp, batched, nbucket) < std::tie(o.kind, o.has_amp, o.batched, o.nbucket);
+        return std::tie(kind, has_amp, batched, split_path, nbucket) <
+               std::tie(o.kind, o.has_amp, o.batched, o.split_path, o.nbucket);
     }
 };
+
+static bool has_tile(const std::vector<int>& tiles, int v)
+{
+    return std::find(tiles.begin(), tiles.end(), v) != tiles.end();
+}
 
 struct Choice {
     int tile = 2;
@@ -362,14 +371,16 @@ int heuristic_split(const idisim_handle* h, int64_t n, int tile)
 
 int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
 {
-    const TuneKey key{c.kind, c.amp != nullptr, 0, nbucket(c.n)};
+    const TuneKey key{c.kind, c.amp != nullptr, 0, c.split_path ? 1 : 0, nbucket(c.n)};
     Choice ch;
     auto it = h->tuned.find(key);
     const bool forced = h->force_tile >= 0 || h->force_split > 0;
     if (forced) {
+        if (h->force_tile >= 0 && !has_tile(tiles, h->force_tile))
+            return fail(IDISIM_E_UNSUPPORTED, "tile variant " + std::to_string(h->force_tile) + " has no FP64 kernel");
         ch.tile = h->force_tile >= 0 ? h->force_tile : tiles.front();
         ch.split = h->force_split > 0 ? h->force_split : heuristic_split(h, c.n, ch.tile);
-    } else if (it != h->tuned.end()) {
+    } else if (it != h->tuned.end() && has_tile(tiles, it->second.tile)) {
         ch = it->second;
     } else if (h->autotune) {
         // time candidates on (at most) ~2e10 pairs of this call's work
@@ -410,7 +421,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
         // heuristic: 2x2 pixels per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
         for (int v : {kTileMfma, 2, 1, 3, 0})
-            if (std::find(tiles.begin(), tiles.end(), v) != tiles.end()) {
+            if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
             }
@@ -960,13 +971,16 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     }
     BatchCall c{emitters, amplitudes, out, out_kind, st, split_path};
 
-    const TuneKey key{out_kind, amplitudes != nullptr, 1, nbucket(std::max<int64_t>(1, total / nshots))};
+    const TuneKey key{out_kind, amplitudes != nullptr, 1, split_path ? 1 : 0,
+                      nbucket(std::max<int64_t>(1, total / nshots))};
     Choice ch;
     auto it = h->tuned.find(key);
     if (h->force_tile >= 0 || h->force_split > 0) {
+        if (h->force_tile >= 0 && !has_tile(tiles, h->force_tile))
+            return fail(IDISIM_E_UNSUPPORTED, "tile variant " + std::to_string(h->force_tile) + " has no FP64 kernel");
         ch.tile = h->force_tile >= 0 ? h->force_tile : tiles.front();
         ch.split = h->force_split > 0 ? h->force_split : occupancy(h, c, ch.tile);
-    } else if (it != h->tuned.end()) {
+    } else if (it != h->tuned.end() && has_tile(tiles, it->second.tile)) {
         ch = it->second;
     } else if (h->autotune) {
         Timer tm;
@@ -999,7 +1013,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     } else {
         ch.tile = tiles.front();
         for (int v : {kTileMfma, 2, 1, 3, 0})
-            if (std::find(tiles.begin(), tiles.end(), v) != tiles.end()) {
+            if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
             }

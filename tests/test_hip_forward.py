@@ -222,3 +222,23 @@ def test_fallback_and_strict_precision(lib, oracle_mod):
     with pytest.raises(lib.IdisimError) as ei:
         h.forward(torch.from_numpy(at).cuda())
     assert ei.value.code == lib.E_PRECISION
+
+
+def test_autotune_cache_across_extents(lib, oracle_mod):
+    """One autotuning handle, the same emitter count at three extents: a 1 nm ball (FP32-split
+    path, tuned and cached), a 10 um ball (no tile meets the bound: FP64 fallback) and the 1 nm
+    ball again.  A cached choice is reused only for the kernel family and tiles it was tuned for."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(8)
+    k = 2 * np.pi / (1.24 / 100)
+    g = Geometry(4, 4, 10.0, 1000.0, k, nf=True)
+    small = atoms_ball(rng, 100, 0.001)
+    large = atoms_ball(rng, 100, 10.0)
+    h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+    h.set_option(lib.OPT_AUTOTUNE, 1)
+    for at, fb in ((small, 0), (large, 1), (small, 0)):
+        f, _ = run(lib, g, at, handle=h)
+        assert h.plan()["fallback"] == fb
+        tol = SPLIT_RTOL if fb == 0 else max(fp64_tol(g), 1e-6)
+        assert rel_l2(f, oracle_mod.field_c(g, at)) <= tol
