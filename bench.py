@@ -16,7 +16,8 @@ the CPU restatement on the full image), batched (256 shots, N_s ~ U{1000..50000}
 30 nm ball, 256^2 px of 100 um at 0.1 m, one persistent graph-replayed launch),
 large (1e6 emitters in three balls, amplitudes U[0.5, 1], 1024^2 px of 50 um at
 0.5 m, 16 shots), adjoint (4096 emitters, 256^2 px of 75 um at 0.1 m), timedep (time-dependent
-simulation: 10000 emitters, 128^2 px, near field, float32 arrival keys).
+simulation: 10000 emitters, 128^2 px, near field, float32 arrival keys), e2e (simulate() on the
+default workload's object per image: host or device draw, copies and kernel).
 
 Prints ONE JSON line (rank 0).  --gpus must be 1: this build runs on one GPU.
 """
@@ -453,12 +454,55 @@ def bench_timedep(args):
                        "pairs_per_step": pairs}}
 
 
+def bench_e2e(args):
+    """End-to-end simulate() on the default workload's object: each image draws 100000 of the
+    155331 simple-cubic sites and fresh phases, runs the FP32-split kernel and returns the
+    complex64 field to the host.  Host draw (numpy, as the reference's simobj does) and device
+    draw (device_rng=True: Philox choice on the GPU) are both timed; the rate includes the draw,
+    the PCIe copies and the launches."""
+    import torch
+
+    from idi_amd.simulation import hip, simobj
+
+    m = int(np.ceil(0.01 / 3e-4))
+    g = np.arange(-m, m + 1) * 3e-4
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pts = pts[np.linalg.norm(pts, axis=1) <= 0.01 * (1 + 1e-12)]
+    obj = simobj.sc(1.24 / LAMBDA_UM, 100_000, 3e-4, repeats=[2 * m + 1] * 3)
+    obj._allpos = pts
+    obj.rng = np.random.default_rng(2001)
+    n0 = 512
+    pairs = obj.N * n0 * n0
+    kw = dict(settings="single nf", verbose=False)
+    rates = {}
+    for name, dev in (("host_draw", False), ("device_draw", True)):
+        hip.simulate(max(1, args.warmup), obj, n0, 75.0, 2e5, device_rng=dev, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = hip.simulate(args.steps, obj, n0, 75.0, 2e5, device_rng=dev, **kw)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        assert res.shape == (args.steps, n0, n0)
+        rates[name] = {"value": pairs * args.steps / wall, "ms_per_image": wall * 1e3 / args.steps}
+        log(name, rates[name])
+    h = hip.get_handle(n0, 75.0, 2e5, obj.k, "single nf")
+    return {"metric": METRIC + " [end-to-end simulate(): draw, copies, kernel]",
+            "value": rates["host_draw"]["value"], "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": rates["host_draw"]["ms_per_image"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "simulate(Nimg, sc crystal clipped to a 10 nm ball: 100000 of 155331 sites "
+                                   "per image, random phases), 512x512 px of 75 um at z=0.2 m, settings "
+                                   "'single nf', complex64 field returned to the host per image",
+                       "pairs_per_step": pairs, "sites": int(len(pts)), "plan": h.plan()},
+            "device_draw": rates["device_draw"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep"])
+    ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep", "e2e"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
@@ -476,7 +520,7 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     res = {"single": bench_single, "batched": bench_batched, "large": bench_large,
-           "adjoint": bench_adjoint, "timedep": bench_timedep, "c1": bench_c1}[args.workload](args)
+           "adjoint": bench_adjoint, "timedep": bench_timedep, "c1": bench_c1, "e2e": bench_e2e}[args.workload](args)
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.rand(n, device="cuda")

@@ -110,15 +110,17 @@ class _Staging:
         self.ready = None  # event after the H2D copy; the host buffer may be rewritten once it has passed
 
     def upload(self, atoms, amp):
+        """Copy one image's emitters (None: they are drawn on the device) and amplitudes
+        (None: none) through the pinned buffers, asynchronously on the current stream."""
         import torch
 
         if self.ready is not None:
             self.ready.synchronize()
-        self.h.numpy()[...] = atoms
+        if atoms is not None:
+            self.h.numpy()[...] = atoms
+            self.d.copy_(self.h, non_blocking=True)
         if amp is not None:
             self.ha.numpy()[...] = amp
-        self.d.copy_(self.h, non_blocking=True)
-        if amp is not None:
             self.da.copy_(self.ha, non_blocking=True)
         self.ready = torch.cuda.Event()
         self.ready.record()
@@ -188,10 +190,11 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                  **kwargs):
     """Generator of simulated wavefields (cuda.py:16-138).
 
-    Two-deep pipeline: while the GPU computes image i, the host draws the
-    emitters of image i+1 and copies them through pinned memory on the same
-    stream; image i is copied back (or cloned on the device) after image i+1
-    has been launched.
+    Two-deep pipeline on one stream: image i is copied back into pinned memory
+    (or cloned on the device), then the host draws the emitters of image i+1
+    while the GPU still computes image i, copies them through pinned memory and
+    launches image i+1; image i is yielded as soon as its copy has landed, so the
+    caller works on it while image i+1 computes.
 
     device_rng=True (extension): ``simobj.sphere``, ``simobj.multisphere`` (ball
     centres and counts still drawn on the host), ``simobj.gauss`` / ``gnorm``
@@ -244,19 +247,27 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                     else:
                         ext = _device_draw(source, simobject, st.d, seed, draw_cache)
                     if amp is not None:
-                        st.da.copy_(torch.from_numpy(amp))
+                        st.upload(None, amp)
                 h.forward(st.d, st.da if amp is not None else None, outs[slot], kind, ext)
 
             launch(0, a0)
             slot = 0
             while count < maximg:
                 count += 1
-                if count < maximg:
-                    launch(1 - slot, _amplitudes(simobject, amplitudes))
+                # image `count` is copied out before the next image is launched: the
+                # consumer gets it while the GPU computes the next one
                 if returngpu:
                     ret = outs[slot].clone()
                 else:
-                    ret = outs[slot].cpu().numpy()
+                    host = torch.empty((n0, n1), dtype=dtype, pin_memory=True)
+                    host.copy_(outs[slot], non_blocking=True)
+                    copied = torch.cuda.Event()
+                    copied.record()
+                if count < maximg:
+                    launch(1 - slot, _amplitudes(simobject, amplitudes))
+                if not returngpu:
+                    copied.synchronize()
+                    ret = host.numpy()
                 yield ret
                 slot = 1 - slot
 
