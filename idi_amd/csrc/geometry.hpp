@@ -162,10 +162,19 @@ struct WorkItem {
 // Pixel-tile variants.  0..3: VALU kernels, lanes form a 16 x 16 grid and each
 // lane owns PX x PY pixels (16x16, 16x32, 32x32, 32x64 pixels per workgroup).
 // 4: MFMA kernel (kernels_mfma.hip), 16 x 16 pixels per workgroup.
-constexpr int kNumTiles = 5;
+// 5, 6: mirror-pair VALU kernels (one or two 2 x 2 quads per lane; 32x32 and
+// 32x64 pixels per workgroup), which share one sincos per quad and emitter.
+constexpr int kNumTiles = 7;
 constexpr int kTileMfma = 4;
+constexpr int kTilePair1 = 5;
+constexpr int kTilePair2 = 6;
+__host__ __device__ constexpr bool tile_pair(int v) { return v == kTilePair1 || v == kTilePair2; }
+__host__ __device__ constexpr int pair_quads(int v) { return v == kTilePair2 ? 2 : 1; }
 __host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileMfma ? 2 : 1; }
-__host__ __device__ constexpr int tile_py(int v) { return v == 0 || v == kTileMfma ? 1 : (v == 3 ? 4 : 2); }
+__host__ __device__ constexpr int tile_py(int v)
+{
+    return v == 0 || v == kTileMfma ? 1 : (v == 3 || v == kTilePair2 ? 4 : 2);
+}
 __host__ __device__ constexpr int tile_rows(int v) { return 16 * tile_px(v); }
 __host__ __device__ constexpr int tile_cols(int v) { return 16 * tile_py(v); }
 

@@ -84,9 +84,10 @@ struct idisim_handle {
     int time_path = 0;
     double tol_rev = 5e-6;
     // planner tables
-    double du_max[kNumTiles] = {0, 0, 0, 0};
-    double dR_max[kNumTiles] = {0, 0, 0, 0};
+    double du_max[kNumTiles] = {};
+    double dR_max[kNumTiles] = {};
     double Rmin = 0.0, Rmax = 0.0;
+    double pair_coef = 0.0;   // mirror-pair remainder per unit emitter extent (rev per length)
     double rho = 0.0;   // emitter extent of the current call
     // device scratch
     void* scratch = nullptr;
@@ -156,6 +157,24 @@ void planner_tables(idisim_handle* h)
         h->du_max[v] = 1.05 * du;
         h->dR_max[v] = 1.05 * dR;
     }
+    // Mirror-pair remainder: for a 2 x 2 quad the mean dq of a pair differs from the
+    // quad mean by 1/4 of the mixed second difference, krev pix^2 |d_a d_b rh| / 4
+    // (plus fourth-order terms).  With orthonormal detector axes a, b and
+    // alpha = rh . a, beta = rh . b:  d_a d_b rh = (3 alpha beta rh - alpha b - beta a) / R^2,
+    // so |d_a d_b rh| <= (sqrt(2) s + 3 s^2) / R^2 with s >= |alpha|, |beta|.  In the
+    // detector frame alpha = dx / R and beta = dy / R (dx, dy the in-plane pixel
+    // coordinates), largest in magnitude at the corners; one pixel is added for the
+    // quads that reach past the last row or column.
+    double emax = 0.0;
+    for (double fx : {-1.0, (double)g.n0}) {
+        for (double fy : {-1.0, (double)g.n1}) {
+            emax = std::max(emax, std::fabs((fx - g.half0) * g.pix + g.off0));
+            emax = std::max(emax, std::fabs((fy - g.half1) * g.pix + g.off1));
+        }
+    }
+    const double sm = std::min(1.0, emax / h->Rmin);
+    h->pair_coef = 1.05 * g.krev * g.pix * g.pix * (std::sqrt(2.0) * sm + 3.0 * sm * sm) /
+                   (4.0 * h->Rmin * h->Rmin);
 }
 
 // Error (revolutions) of folding W_cj into A_cj, i.e. of replacing lam_p W_cj
@@ -209,7 +228,16 @@ double plan_bound(const idisim_handle* h, int v, double rho)
     }
     e += kSinCosErrRev;
     const double fe = fold_error(h, v, rho);
-    if (fe <= 0.1 * h->tol_rev) e += fe;  // W folded into A
+    if (fe <= 0.1 * h->tol_rev) {
+        e += fe;  // W folded into A
+    } else if (tile_pair(v) && (g.nf || g.secondorder)) {
+        return std::numeric_limits<double>::infinity();  // mirror pairs need the fold
+    }
+    if (tile_pair(v)) {
+        e += h->pair_coef * rho * (1.0 + sig);  // dropped remainder e_k
+        e += kSinCosErrRev;                     // second sincos (delta_k)
+        e += 2.0 * u24;                         // e^{i tau_c} (cos, sin) delta_k products, as phase
+    }
     return e;
 }
 
@@ -269,7 +297,7 @@ std::vector<int> eligible_tiles(const idisim_handle* h, double rho, double* best
     std::vector<int> out;
     *best_bound = std::numeric_limits<double>::infinity();
     for (int v = 0; v < kNumTiles; ++v) {
-        if (v == kTileMfma && h->summode != 0) continue;
+        if ((v == kTileMfma || tile_pair(v)) && h->summode != 0) continue;
         const double b = plan_bound(h, v, rho);
         *best_bound = std::min(*best_bound, b);
         if (b <= h->tol_rev) out.push_back(v);
@@ -324,6 +352,10 @@ int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
         if (tile == kTileMfma) {
             if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the MFMA tile has no per-term Kahan mode");
             IDISIM_HIP(launch_fwd_mfma(gl, c.emit, c.amp, c.n, splits, partial, c.out, c.kind, c.st));
+        } else if (tile_pair(tile)) {
+            if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles have no per-term Kahan mode");
+            if (w) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles need the curvature folded into A");
+            IDISIM_HIP(launch_fwd_pair(tile, a, gl, c.emit, c.amp, c.n, splits, partial, c.out, c.kind, c.st));
         } else {
             IDISIM_HIP(launch_fwd_split(tile, w, a, h->summode, gl, c.emit, c.amp, c.n, splits, partial, c.out,
                                         c.kind, c.st));
@@ -418,9 +450,9 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
         }
         h->tuned[key] = ch;
     } else {
-        // heuristic: 2x2 pixels per lane if allowed, enough splits for 4 WG/CU
+        // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTilePair1, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -431,6 +463,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     if (ch.split > maxs) ch.split = (int)maxs;
     int rc = run_forward(h, c, ch.tile, ch.split);
     if (rc) return rc;
+    if (c.split_path) h->plan.phase_bound = plan_bound(h, ch.tile, h->rho);  // the tile that ran
     h->plan.tile = ch.tile;
     h->plan.tile_rows = tile_rows(ch.tile);
     h->plan.tile_cols = tile_cols(ch.tile);
@@ -504,6 +537,11 @@ int launch_batched(idisim_handle* h, const BatchCall& c, int tile, int blocks_pe
             if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the MFMA tile has no per-term Kahan mode");
             IDISIM_HIP(launch_batched_mfma(gl, c.emit, c.amp, h->d_items, h->nitems, h->d_counter, nblocks, c.out,
                                            c.kind, st));
+        } else if (tile_pair(tile)) {
+            if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles have no per-term Kahan mode");
+            if (w) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles need the curvature folded into A");
+            IDISIM_HIP(launch_batched_pair(tile, a, gl, c.emit, c.amp, h->d_items, h->nitems, h->d_counter, nblocks,
+                                           c.out, c.kind, st));
         } else {
             IDISIM_HIP(launch_batched_split(tile, w, a, h->summode, gl, c.emit, c.amp, h->d_items, h->nitems,
                                             h->d_counter, nblocks, c.out, c.kind, st));
@@ -521,6 +559,7 @@ int occupancy(idisim_handle* h, const BatchCall& c, int tile)
     if (c.split_path) {
         const bool w = (g.nf || g.secondorder) && fold_error(h, tile, h->rho) > 0.1 * h->tol_rev;
         if (tile == kTileMfma) return occupancy_batched_mfma(!w);
+        if (tile_pair(tile)) return occupancy_batched_pair(tile, (c.amp != nullptr) || (g.nf && g.scale));
         return occupancy_batched_split(tile, w, (c.amp != nullptr) || (g.nf && g.scale), h->summode);
     }
     return occupancy_batched_fp64(tile);
@@ -1012,7 +1051,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTilePair1, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -1023,6 +1062,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     if (rc) return rc;
     rc = run_batched(h, c, ch.tile, ch.split);
     if (rc) return rc;
+    if (split_path) h->plan.phase_bound = plan_bound(h, ch.tile, h->rho);
     h->plan.tile = ch.tile;
     h->plan.tile_rows = tile_rows(ch.tile);
     h->plan.tile_cols = tile_cols(ch.tile);

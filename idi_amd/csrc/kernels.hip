@@ -195,6 +195,238 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP32-split tile with mirror pairs (tile variants 5 and 6).  Each lane owns QY
+// quads of 2 x 2 pixels (a 2 x 2QY block; the 16 x 16 lanes cover 32 x 32QY
+// pixels).  In a quad, pixels (0,0)-(1,1) (pair 0) and (1,0)-(0,1) (pair 1) are
+// mirror images about the quad centre.  With dq the tile-relative phase gradient
+// of a pixel, mc the mean dq of the quad and d_k half the difference of pair k's
+// dq (second member minus first):
+//   tau = tau_c + e_k -/+ delta_k,  tau_c = A + mc . r',  delta_k = d_k . r',
+// where e_k = (mean dq of pair k - mc) . r' is a second-order remainder.  For a
+// square quad the isotropic curvature cancels in e_k and what is left is +/- 1/4
+// of the mixed second difference of dq, which the host planner bounds and which
+// is left out.  Per quad and emitter: one sincos of tau_c; per pair one sincos of
+// delta_k and four FMA into
+//   U_k = sum e^{i tau_c} cos delta_k,   V_k = sum e^{i tau_c} sin delta_k,
+// and the pair's fields are U_k - i V_k (first member) and U_k + i V_k (second).
+// That is 6 transcendentals and 17 FMA per 4 pixels where split_tile spends 8
+// and 20.  The W term must be folded into A (no lam_p per pixel); sums are the
+// chunked Kahan sums of split_tile.
+template <int QY, bool HAS_AMP>
+__device__ __forceinline__ void pair_tile(const Geom& g, const double* __restrict__ emit,
+                                          const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0,
+                                          int ty0, float4* s0, float2* s1, float (&acc)[QY][2][4],
+                                          float (&cmp)[QY][2][4])
+{
+    constexpr int TX = 32, TY = 32 * QY;
+    const int tid = threadIdx.x;
+    const int lx = tid >> 4, ly = tid & 15;
+
+    double ccx, ccy;
+    tile_centre(g, tx0, ty0, TX, TY, ccx, ccy);
+    const Frame c = pixel_frame(g, ccx, ccy);
+
+    float mc[QY][3], d[QY][2][3];
+#pragma unroll
+    for (int q = 0; q < QY; ++q) {
+        double dq[2][2][3];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const Frame f = pixel_frame(g, (double)(tx0 + 2 * lx + i), (double)(ty0 + 2 * QY * ly + 2 * q + j));
+#pragma unroll
+                for (int a = 0; a < 3; ++a) dq[i][j][a] = -g.krev * (f.u[a] - c.u[a]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mc[q][a] = (float)(0.25 * ((dq[0][0][a] + dq[1][1][a]) + (dq[1][0][a] + dq[0][1][a])));
+            d[q][0][a] = (float)(0.5 * (dq[1][1][a] - dq[0][0][a]));
+            d[q][1][a] = (float)(0.5 * (dq[0][1][a] - dq[1][0][a]));
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                acc[q][k][m] = 0.f;
+                cmp[q][k][m] = 0.f;
+            }
+        }
+    }
+
+    for (int64_t base = e0; base < e1; base += kChunk) {
+        const int cnt = (e1 - base) < (int64_t)kChunk ? (int)(e1 - base) : kChunk;
+        __syncthreads();  // the previous chunk has been consumed by every lane
+        if (tid < cnt) {
+            const int64_t e = base + tid;
+            const double* a4 = emit + 4 * e;
+            const double ea = (HAS_AMP && amp != nullptr) ? amp[e] : 1.0;
+            const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], ea);
+            s0[tid] = make_float4(r.x, r.y, r.z, r.A);
+            s1[tid] = make_float2(r.W, r.amp);
+        }
+        __syncthreads();
+
+        float su[QY][2][4];
+#pragma unroll
+        for (int q = 0; q < QY; ++q)
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) su[q][k][m] = 0.f;
+#pragma unroll 2
+        for (int e = 0; e < cnt; ++e) {
+            const float4 r0 = s0[e];
+            const float am = HAS_AMP ? s1[e].y : 1.f;
+#pragma unroll
+            for (int q = 0; q < QY; ++q) {
+                float t = fmaf(mc[q][0], r0.x, r0.w);
+                t = fmaf(mc[q][1], r0.y, t);
+                t = fmaf(mc[q][2], r0.z, t);
+                float sc = __builtin_amdgcn_sinf(t);
+                float cc = __builtin_amdgcn_cosf(t);
+                if (HAS_AMP) {
+                    sc *= am;
+                    cc *= am;
+                }
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    float dl = d[q][k][0] * r0.x;
+                    dl = fmaf(d[q][k][1], r0.y, dl);
+                    dl = fmaf(d[q][k][2], r0.z, dl);
+                    const float sd = __builtin_amdgcn_sinf(dl);
+                    const float cd = __builtin_amdgcn_cosf(dl);
+                    su[q][k][0] = fmaf(cc, cd, su[q][k][0]);
+                    su[q][k][1] = fmaf(sc, cd, su[q][k][1]);
+                    su[q][k][2] = fmaf(cc, sd, su[q][k][2]);
+                    su[q][k][3] = fmaf(sc, sd, su[q][k][3]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QY; ++q)
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) kahan_add(acc[q][k][m], cmp[q][k][m], su[q][k][m]);
+    }
+}
+
+// The four pixels of quad q of a lane: pixel (i, j) of the quad, its field in FP64
+// from the compensated U and V sums of its pair.
+template <int QY>
+__device__ __forceinline__ void pair_pixel(const float (&acc)[QY][2][4], const float (&cmp)[QY][2][4], int q,
+                                           int i, int j, double& re, double& im)
+{
+    // pair 0: (0,0) first, (1,1) second; pair 1: (1,0) first, (0,1) second
+    const int k = (i == j) ? 0 : 1;
+    const bool second = (k == 0) ? (i == 1) : (i == 0);
+    const double ur = (double)acc[q][k][0] - (double)cmp[q][k][0];
+    const double ui = (double)acc[q][k][1] - (double)cmp[q][k][1];
+    const double vr = (double)acc[q][k][2] - (double)cmp[q][k][2];
+    const double vi = (double)acc[q][k][3] - (double)cmp[q][k][3];
+    // (U +/- i V): re = ur -/+ vi, im = ui +/- vr
+    re = second ? ur - vi : ur + vi;
+    im = second ? ui + vr : ui - vr;
+}
+
+// An FP64 value as the FP32 pair (hi, lo) with hi - lo ~ v, the form of the
+// split partials (the finalize kernel forms hi - lo in FP64).
+__device__ __forceinline__ void split_hi_lo(double v, float& hi, float& lo)
+{
+    hi = (float)v;
+    lo = (float)((double)hi - v);
+}
+
+template <int QY, bool HAS_AMP>
+__global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __restrict__ emit,
+                                                     const double* __restrict__ amp, int64_t n, int tiles_y,
+                                                     int64_t per_split, float4* __restrict__ partial, void* out,
+                                                     int kind)
+{
+    constexpr int TX = 32, TY = 32 * QY;
+    __shared__ float4 s0[kChunk];
+    __shared__ float2 s1[kChunk];
+    const int tile = blockIdx.x;
+    const int split = blockIdx.y;
+    const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
+    const int64_t e0 = (int64_t)split * per_split;
+    const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
+    float acc[QY][2][4], cmp[QY][2][4];
+    pair_tile<QY, HAS_AMP>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, acc, cmp);
+
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+#pragma unroll
+    for (int q = 0; q < QY; ++q) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int x = tx0 + 2 * lx + i, y = ty0 + 2 * QY * ly + 2 * q + j;
+                if (x < g.n0 && y < g.n1) {
+                    double re, im;
+                    pair_pixel<QY>(acc, cmp, q, i, j, re, im);
+                    const size_t idx = (size_t)x * g.n1 + y;
+                    if (partial != nullptr) {
+                        float rh, rl, ih, il;
+                        split_hi_lo(re, rh, rl);
+                        split_hi_lo(im, ih, il);
+                        partial[(size_t)split * npix + idx] = make_float4(rh, ih, rl, il);
+                    } else {
+                        store_split(g, x, y, re, im, out, kind, idx);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Persistent batched form of k_fwd_pair (the queue protocol of k_batched_split).
+template <int QY, bool HAS_AMP>
+__global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* __restrict__ emit,
+                                                         const double* __restrict__ amp,
+                                                         const WorkItem* __restrict__ items, int nitems,
+                                                         int* __restrict__ counter, int tiles_y, void* out, int kind)
+{
+    constexpr int TX = 32, TY = 32 * QY;
+    __shared__ float4 s0[kChunk];
+    __shared__ float2 s1[kChunk];
+    __shared__ int s_item;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nitems) break;
+        const WorkItem w = items[it];
+        const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
+        float acc[QY][2][4], cmp[QY][2][4];
+        pair_tile<QY, HAS_AMP>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, acc, cmp);
+        char* outs = static_cast<char*>(out) +
+                     (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
+#pragma unroll
+        for (int q = 0; q < QY; ++q) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int x = tx0 + 2 * lx + i, y = ty0 + 2 * QY * ly + 2 * q + j;
+                    if (x < g.n0 && y < g.n1) {
+                        double re, im;
+                        pair_pixel<QY>(acc, cmp, q, i, j, re, im);
+                        store_split(g, x, y, re, im, outs, kind, (size_t)x * g.n1 + y);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // FP64 combine of the split partials: a workgroup owns 64 pixels, its four rows of
 // threads stride over the splits (several loads in flight per pixel) and the rows are
 // summed through LDS.
@@ -529,6 +761,77 @@ int occupancy_batched_split(int v, bool w, bool a, int s)
 {
     IDISIM_DISPATCH_SPLIT(occupancy_split_v)
     return 1;
+}
+
+template <int QY, bool A>
+static hipError_t launch_pair_q(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
+                                float4* partial, void* out, int kind, hipStream_t st)
+{
+    const int tiles_x = (g.n0 + 31) / 32;
+    const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
+    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    dim3 grid(tiles_x * tiles_y, splits);
+    hipLaunchKernelGGL((k_fwd_pair<QY, A>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
+                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_fwd_pair(int v, bool a, const Geom& g, const double* emit, const double* amp, int64_t n,
+                           int splits, float4* partial, void* out, int kind, hipStream_t st)
+{
+    if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;  // needs W folded into A
+    switch ((pair_quads(v) << 1) | (int)a) {
+        case 2: return launch_pair_q<1, false>(g, emit, amp, n, splits, partial, out, kind, st);
+        case 3: return launch_pair_q<1, true>(g, emit, amp, n, splits, partial, out, kind, st);
+        case 4: return launch_pair_q<2, false>(g, emit, amp, n, splits, partial, out, kind, st);
+        case 5: return launch_pair_q<2, true>(g, emit, amp, n, splits, partial, out, kind, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int QY, bool A>
+static hipError_t launch_batched_pair_q(const Geom& g, const double* emit, const double* amp,
+                                        const WorkItem* items, int nitems, int* counter, int nblocks, void* out,
+                                        int kind, hipStream_t st)
+{
+    const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL((k_batched_pair<QY, A>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp, items, nitems,
+                       counter, tiles_y, out, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_batched_pair(int v, bool a, const Geom& g, const double* emit, const double* amp,
+                               const WorkItem* items, int nitems, int* counter, int nblocks, void* out, int kind,
+                               hipStream_t st)
+{
+    if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;
+    switch ((pair_quads(v) << 1) | (int)a) {
+        case 2: return launch_batched_pair_q<1, false>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        case 3: return launch_batched_pair_q<1, true>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        case 4: return launch_batched_pair_q<2, false>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        case 5: return launch_batched_pair_q<2, true>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int QY, bool A>
+static int occupancy_pair_q()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_pair<QY, A>, kBlock, 0) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+int occupancy_batched_pair(int v, bool a)
+{
+    switch ((pair_quads(v) << 1) | (int)a) {
+        case 2: return occupancy_pair_q<1, false>();
+        case 3: return occupancy_pair_q<1, true>();
+        case 4: return occupancy_pair_q<2, false>();
+        case 5: return occupancy_pair_q<2, true>();
+        default: return 1;
+    }
 }
 
 hipError_t launch_finalize_split(const Geom& g, const float4* partial, int splits, void* out, int kind,

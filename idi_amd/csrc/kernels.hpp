@@ -18,6 +18,14 @@ hipError_t launch_batched_split(int v, bool w, bool a, int s, const Geom& g, con
                                 const double* amp, const WorkItem* items, int nitems, int* counter, int nblocks,
                                 void* out, int kind, hipStream_t st);
 int occupancy_batched_split(int v, bool w, bool a, int s);
+// Mirror-pair tiles (tile_pair(v)); W must be folded into A (g.foldw) when the
+// phase model has a curvature / second-order term.
+hipError_t launch_fwd_pair(int v, bool a, const Geom& g, const double* emit, const double* amp, int64_t n,
+                           int splits, float4* partial, void* out, int kind, hipStream_t st);
+hipError_t launch_batched_pair(int v, bool a, const Geom& g, const double* emit, const double* amp,
+                               const WorkItem* items, int nitems, int* counter, int nblocks, void* out, int kind,
+                               hipStream_t st);
+int occupancy_batched_pair(int v, bool a);
 
 // FP32-split forward with the phase on the matrix cores (tile variant kTileMfma).
 hipError_t launch_fwd_mfma(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,

@@ -69,7 +69,10 @@ extern "C" {
 
 /* options (idisim_set_option) */
 #define IDISIM_OPT_AUTOTUNE     1 /* 1 (default): time candidate plans on first use */
-#define IDISIM_OPT_TILE         2 /* force tile variant 0..3 (-1 = planner)         */
+#define IDISIM_OPT_TILE         2 /* force a tile variant (-1 = planner): 0..3 VALU
+                                     (16x16 .. 32x64 px), 4 MFMA (FP32-split only),
+                                     5, 6 mirror pairs (FP32-split only, 32x32 and
+                                     32x64 px); FP64 handles take 0..3             */
 #define IDISIM_OPT_SPLIT        3 /* force emitter split count (0 = planner)        */
 #define IDISIM_OPT_SUMMODE      4 /* 0 (default): per-chunk partials merged with
                                      Kahan; 1: Kahan on every term (cpu.py:97-105) */

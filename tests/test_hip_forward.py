@@ -242,3 +242,51 @@ def test_autotune_cache_across_extents(lib, oracle_mod):
         assert h.plan()["fallback"] == fb
         tol = SPLIT_RTOL if fb == 0 else max(fp64_tol(g), 1e-6)
         assert rel_l2(f, oracle_mod.field_c(g, at)) <= tol
+
+
+@pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder", "nf_scale_amp"])
+def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
+    """Mirror-pair tiles (5: one 2 x 2 quad per lane, 6: two) on the pixel pitch and distance of
+    the 100k-site image (75 um at 0.2 m, sites of a 10 nm ball), on an odd detector so that quads
+    reach past the last row and column; every emitter split and output kind."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(45)
+    g = Geometry(37, 53, 75.0, 2e5, K_UM, nf=mode.startswith("nf"), scale="scale" in mode,
+                 secondorder="secondorder" in mode)
+    at, _ = sc_ball(rng, 3000, 3e-4, 0.01)
+    amp = rng.uniform(0.5, 1.0, len(at)) if "amp" in mode else None
+    ref = oracle_mod.field_c(g, at, amp)
+    for tile in (5, 6):
+        for split in (1, 3):
+            f, h = run(lib, g, at, amp, tile=tile, split=split)
+            p = h.plan()
+            assert (p["tile"], p["split"]) == (tile, split)
+            assert p["phase_bound"] <= 5e-6
+            assert rel_l2(f, ref) <= SPLIT_RTOL, (tile, split, rel_l2(f, ref))
+            assert rel_l2(intensity(f), intensity(ref)) <= SPLIT_RTOL
+        I, _ = run(lib, g, at, amp, kind=lib.OUT_INTENSITY, tile=tile, split=2)
+        assert rel_l2(I, intensity(ref)) <= SPLIT_RTOL
+
+
+def test_pair_tiles_eligibility(lib, oracle_mod):
+    """Pixels of 100 um at 0.1 m and a 30 nm ball: the pair remainder (~1.2e-5 rev) exceeds the
+    5e-6 rev tolerance, so the planner never picks a pair tile and a forced one reports its bound;
+    the per-term Kahan mode has no pair kernel."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(46)
+    # the corner region of a 256^2 detector: offset by 96 pixels
+    g = Geometry(64, 64, 100.0, 1e5, K_UM, detoffset=(9600.0, 9600.0), nf=True)
+    at = atoms_ball(rng, 600, 0.03)
+    _, h = run(lib, g, at)
+    assert h.plan()["tile"] not in (5, 6)
+    f, h = run(lib, g, at, tile=5, split=1)
+    assert h.plan()["phase_bound"] > 5e-6
+    with pytest.raises(lib.IdisimError):
+        run(lib, g, at, tile=5, split=1, summode=1)
+    # the same detector at 0.5 m: eligible, and within the bound of the oracle
+    g2 = Geometry(64, 64, 100.0, 5e5, K_UM, detoffset=(9600.0, 9600.0), nf=True)
+    f, h = run(lib, g2, at, tile=6, split=1)
+    assert h.plan()["phase_bound"] <= 5e-6
+    assert rel_l2(f, oracle_mod.field_c(g2, at)) <= SPLIT_RTOL

@@ -1,0 +1,96 @@
+"""The mirror-pair remainder bound of the FP32-split planner, checked by brute force.
+
+In a 2 x 2 pixel quad the mean phase gradient of each mirror pair differs from the
+quad mean by a second-order remainder.  The planner bounds it per unit emitter
+extent by
+    1.05 krev pix^2 (sqrt(2) s + 3 s^2) / (4 Rmin^2),   s = max(|dx|, |dy|) / Rmin
+(dx, dy: in-plane pixel coordinates, one pixel past the detector edge included).
+Here the remainder is evaluated exactly in FP64 for every quad of several
+detectors (plain, offset, rotated, odd sizes) and must stay below the bound; the
+bound must also stay within a modest factor of the truth, so that a dropped term
+that made it vacuous would show.
+"""
+import numpy as np
+import pytest
+
+from conftest import K_UM
+
+
+def _unit_dirs(g, xs, ys):
+    """rh = r_p / |r_p| at pixel indices xs (rows) x ys (cols), FP64, in the lab frame."""
+    detx = (xs[:, None] - g.n0 / 2) * g.pixelsize + g.detoffset[0]
+    dety = (ys[None, :] - g.n1 / 2) * g.pixelsize + g.detoffset[1]
+    detx, dety = np.broadcast_arrays(detx, dety)
+    v = np.stack([detx, dety, np.full(detx.shape, float(g.detz))])
+    if g.rot:
+        v = np.einsum("ij,jab->iab", g.M, v)
+    return v / np.linalg.norm(v, axis=0)
+
+
+def _bound_coef(g):
+    ex = max(abs((fx - g.n0 / 2) * g.pixelsize + g.detoffset[0]) for fx in (-1.0, float(g.n0)))
+    ey = max(abs((fy - g.n1 / 2) * g.pixelsize + g.detoffset[1]) for fy in (-1.0, float(g.n1)))
+    xs = np.arange(g.n0, dtype=np.float64)
+    ys = np.arange(g.n1, dtype=np.float64)
+    detx = (xs[:, None] - g.n0 / 2) * g.pixelsize + g.detoffset[0]
+    dety = (ys[None, :] - g.n1 / 2) * g.pixelsize + g.detoffset[1]
+    rmin = float(np.sqrt(detx**2 + dety**2 + g.detz**2).min())  # |r_p| is rotation invariant
+    s = min(1.0, max(ex, ey) / rmin)
+    krev = g.k / (2 * np.pi)
+    return 1.05 * krev * g.pixelsize**2 * (np.sqrt(2) * s + 3 * s * s) / (4 * rmin**2)
+
+
+def _true_coef(g):
+    """max over quads and pairs of krev |mean dq of the pair - mean dq of the quad|."""
+    n0q, n1q = (g.n0 + 1) // 2, (g.n1 + 1) // 2
+    xs = np.arange(2 * n0q, dtype=np.float64)
+    ys = np.arange(2 * n1q, dtype=np.float64)
+    u = _unit_dirs(g, xs, ys)  # (3, 2 n0q, 2 n1q); u = rh - z_hat, the constant cancels
+    a = u[:, 0::2, 0::2]
+    b = u[:, 0::2, 1::2]
+    c = u[:, 1::2, 0::2]
+    d = u[:, 1::2, 1::2]
+    mc = 0.25 * ((a + d) + (b + c))
+    r0 = np.linalg.norm(0.5 * (a + d) - mc, axis=0)
+    r1 = np.linalg.norm(0.5 * (b + c) - mc, axis=0)
+    return g.k / (2 * np.pi) * float(max(r0.max(), r1.max()))
+
+
+# (geometry, largest allowed bound / truth): the detectors of the benchmark workloads
+# must be bounded tightly; wide-angle, offset and rotated detectors only soundly.
+CASES = [
+    (dict(n0=512, n1=512, pixelsize=75.0, detz=2e5), 2.0),           # 100k sc sites image
+    (dict(n0=256, n1=256, pixelsize=100.0, detz=1e5), 2.0),          # batched shots
+    (dict(n0=256, n1=256, pixelsize=75.0, detz=1e5), 2.0),           # adjoint image
+    (dict(n0=1024, n1=1024, pixelsize=50.0, detz=5e5), 2.0),         # three balls
+    (dict(n0=37, n1=53, pixelsize=200.0, detz=3e4, detoffset=(1500.0, -800.0)), 8.0),
+    (dict(n0=64, n1=48, pixelsize=150.0, detz=5e4, detangles=(0.3, -0.2)), 8.0),
+    (dict(n0=33, n1=31, pixelsize=1000.0, detz=1e4, detangles=(0.6, 0.4), detoffset=(-2000.0, 900.0)), 30.0),
+]
+
+
+@pytest.mark.parametrize("args,slack", CASES)
+def test_pair_remainder_bound(args, slack):
+    from oracle import Geometry
+
+    g = Geometry(args["n0"], args["n1"], args["pixelsize"], args["detz"], K_UM,
+                 detoffset=args.get("detoffset", (0.0, 0.0)), detangles=args.get("detangles", (0.0, 0.0)))
+    true, bound = _true_coef(g), _bound_coef(g)
+    assert true <= bound, (true, bound)
+    assert bound <= slack * true, (true, bound)
+
+
+def test_pair_remainder_isotropic_part_cancels():
+    """A quad centred on the optical axis (2 x 2 pixels offset by half a pixel): the mixed
+    derivative vanishes there and the isotropic curvature cancels between the two pairs, so
+    the remainder is fourth order, far below the curvature scale krev pix^2 / R^2 that a
+    non-square pairing would leave."""
+    from oracle import Geometry
+
+    g = Geometry(2, 2, 75.0, 2e5, K_UM, detoffset=(37.5, 37.5))
+    krev = K_UM / (2 * np.pi)
+    assert _true_coef(g) < 1e-6 * krev * g.pixelsize**2 / g.detz**2
+    # the same quad paired as rows (a 1 x 4 strip would be) keeps the curvature
+    u = _unit_dirs(g, np.arange(2.0), np.arange(2.0))
+    row_pair = 0.5 * (u[:, 0, 0] + u[:, 0, 1]) - 0.25 * u.sum(axis=(1, 2))
+    assert krev * np.linalg.norm(row_pair) > 0.1 * krev * g.pixelsize**2 / g.detz**2
