@@ -181,13 +181,15 @@ def bench_single(args):
     traffic = args.traffic
     if traffic is None:
         # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE for this plan
-        try:
-            with open(os.path.join(ROOT, "profiles", "fwd_split_sc100k_512px_traffic.json")) as f:
-                rec = json.load(f)
-            if rec["plan"]["tile"] == plan["tile"] and rec["plan"]["split"] == plan["split"]:
-                traffic = rec["hbm_bytes_per_launch"]
-        except (OSError, KeyError, ValueError):
-            traffic = None
+        for name in ("fwd_pair_sc100k_512px_traffic.json", "fwd_split_sc100k_512px_traffic.json"):
+            try:
+                with open(os.path.join(ROOT, "profiles", name)) as f:
+                    rec = json.load(f)
+                if rec["plan"]["tile"] == plan["tile"] and rec["plan"]["split"] == plan["split"]:
+                    traffic = rec["hbm_bytes_per_launch"]
+                    break
+            except (OSError, KeyError, ValueError):
+                continue
     result = {
         "metric": METRIC,
         "value": value,
