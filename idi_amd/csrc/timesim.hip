@@ -66,6 +66,23 @@ __device__ inline void arrival(const Geom& g, const double r[3], double R, const
     *re = cs * scale;
     *im = sn * scale;
 }
+
+// The arrival time alone (the same operations as arrival(); its unused sincos is dead code).
+__device__ inline double arrival_time(const Geom& g, const double r[3], double R, const double* e, double T,
+                                      double c)
+{
+    double t, re, im;
+    arrival(g, r, R, e, T, c, &t, &re, &im);
+    return t;
+}
+
+// The amplitude alone (the same operations as arrival()).
+__device__ inline void arrival_amp(const Geom& g, const double r[3], double R, const double* e, double* re,
+                                   double* im)
+{
+    double t;
+    arrival(g, r, R, e, 0.0, 1.0, &t, re, im);
+}
 }  // namespace
 
 template <typename Tt>
@@ -220,12 +237,13 @@ __global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __rest
 {
     using Sort = rocprim::block_radix_sort<Tt, 256, ITEMS, int>;
     constexpr int S = ITEMS + 1;  // padded stride of the sorted arrays
+    static_assert(256 * ITEMS <= 65536, "sorted indices are stored as 16 bits");
     union Lds {
         typename Sort::storage_type sort;
         Tt kbuf[256 * ITEMS];
         struct {
             Tt key[256 * S];
-            int idx[256 * S];
+            unsigned short idx[256 * S];
         } sorted;
     };
     __shared__ Lds u;
@@ -243,11 +261,7 @@ __global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __rest
 #pragma unroll 1
     for (int j = tid; j < 256 * ITEMS; j += 256) {
         Tt key = (Tt)INFINITY;
-        if (j < n) {
-            double t, re, im;
-            arrival(g, r, R, emit + 4 * (int64_t)j, times[j], c, &t, &re, &im);
-            key = (Tt)t;
-        }
+        if (j < n) key = (Tt)arrival_time(g, r, R, emit + 4 * (int64_t)j, times[j], c);
         u.kbuf[j] = key;
     }
     __syncthreads();
@@ -264,27 +278,52 @@ __global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __rest
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         u.sorted.key[tid * S + k] = keys[k];
-        u.sorted.idx[tid * S + k] = js[k];
+        u.sorted.idx[tid * S + k] = (unsigned short)js[k];
     }
+    __syncthreads();  // every lane reads its neighbours' first and last keys
     const Tt* skey = u.sorted.key + tid * S;
-    const int* sidx = u.sorted.idx + tid * S;
+    const unsigned short* sidx = u.sorted.idx + tid * S;
     const int64_t i0 = (int64_t)tid * ITEMS;
     const int cnt = (int)((n - i0) < ITEMS ? ((n - i0) > 0 ? n - i0 : 0) : ITEMS);
 
-    // lane aggregate over this lane's sorted arrivals (its own LDS entries)
-    DecayState mine = {0.0, 0.0, 0.0, 0};
+    // One pass over this lane's sorted arrivals.  With x_in the field just after the
+    // previous arrival (time tprev, known only after the scan) and P_k the lane's own
+    // field (x_in = 0), x_k = P_k + D_k x_in with D_k = e^{-(t_k - tprev)/tau}, so
+    //   sum_k w_k |x_k|^2 = A + 2 Re(conj(x_in) B) + |x_in|^2 C,
+    //   A = sum w |P|^2,  B = sum w D P,  C = sum w D^2,
+    // and the lane's aggregate (P_last, t_last) feeds the scan.  The previous lane is
+    // full whenever this one has arrivals, so its last key is tprev.
+    const double tprev = (cnt > 0) ? (double)((i0 > 0) ? u.sorted.key[(tid - 1) * S + ITEMS - 1] : skey[0]) : 0.0;
+    double pr = 0.0, pi = 0.0, D = 1.0, tl = tprev;
+    double A = 0.0, Br = 0.0, Bi = 0.0, C = 0.0;
 #pragma unroll 1
     for (int k = 0; k < cnt; ++k) {
         const int j = sidx[k];
-        double t, re, im;
-        arrival(g, r, R, emit + 4 * (int64_t)j, times[j], c, &t, &re, &im);
-        mine = combine(mine, {re, im, (double)skey[k], 1}, inv_tau);
+        double re, im;
+        arrival_amp(g, r, R, emit + 4 * (int64_t)j, &re, &im);
+        const double tk = (double)skey[k];
+        const double dk = exp(-(tk - tl) * inv_tau);
+        pr = re + dk * pr;
+        pi = im + dk * pi;
+        D *= dk;
+        double w = 1.0;
+        if (i0 + k + 1 < n) {
+            const double tn = (k + 1 < ITEMS) ? (double)skey[k + 1] : (double)u.sorted.key[(tid + 1) * S];
+            w = -expm1((tk - tn) * two_over_tau);
+        }
+        A += w * (pr * pr + pi * pi);
+        const double wD = w * D;
+        Br += wD * pr;
+        Bi += wD * pi;
+        C += wD * D;
+        tl = tk;
     }
+    const DecayState mine = {pr, pi, tl, cnt > 0 ? 1 : 0};
     sre[tid] = mine.re;
     sim[tid] = mine.im;
     st[tid] = mine.t;
     shas[tid] = mine.has;
-    __syncthreads();  // also publishes every lane's sorted entries
+    __syncthreads();
     DecayState inc = mine;
     for (int off = 1; off < 256; off <<= 1) {
         DecayState left = {0.0, 0.0, 0.0, 0};
@@ -297,25 +336,12 @@ __global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __rest
         shas[tid] = inc.has;
         __syncthreads();
     }
-    DecayState x = {0.0, 0.0, 0.0, 0};
-    if (tid > 0) x = {sre[tid - 1], sim[tid - 1], st[tid - 1], shas[tid - 1]};
-
-    double acc = 0.0;
-#pragma unroll 1
-    for (int k = 0; k < cnt; ++k) {
-        const int j = sidx[k];
-        double t, re, im;
-        arrival(g, r, R, emit + 4 * (int64_t)j, times[j], c, &t, &re, &im);
-        const double tk = (double)skey[k];
-        x = combine(x, {re, im, tk, 1}, inv_tau);
-        const double m2 = x.re * x.re + x.im * x.im;
-        if (i0 + k + 1 < n) {
-            const double tn = (k + 1 < ITEMS) ? (double)skey[k + 1] : (double)u.sorted.key[(tid + 1) * S];
-            acc += -m2 * expm1((tk - tn) * two_over_tau);
-        } else {
-            acc += m2;
-        }
+    double xr = 0.0, xi = 0.0;
+    if (tid > 0 && shas[tid - 1]) {
+        xr = sre[tid - 1];
+        xi = sim[tid - 1];
     }
+    const double acc = A + 2.0 * (xr * Br + xi * Bi) + (xr * xr + xi * xi) * C;
     ssum[tid] = acc;
     __syncthreads();
     for (int s2 = 128; s2 > 0; s2 >>= 1) {
