@@ -244,16 +244,19 @@ def test_autotune_cache_across_extents(lib, oracle_mod):
         assert rel_l2(f, oracle_mod.field_c(g, at)) <= tol
 
 
-@pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder", "nf_scale_amp"])
+@pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder", "nf_scale_amp", "nf_rot", "ff_rot"])
 def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
     """Mirror-pair tiles (5: one 2 x 2 quad per lane, 6: two) on the pixel pitch and distance of
     the 100k-site image (75 um at 0.2 m, sites of a 10 nm ball), on an odd detector so that quads
-    reach past the last row and column; every emitter split and output kind."""
+    reach past the last row and column; every emitter split and output kind; a rotated and
+    offset detector."""
     from oracle import Geometry
 
     rng = np.random.default_rng(45)
-    g = Geometry(37, 53, 75.0, 2e5, K_UM, nf=mode.startswith("nf"), scale="scale" in mode,
-                 secondorder="secondorder" in mode)
+    rot = mode.endswith("_rot")
+    g = Geometry(37, 53, 75.0, 2e5, K_UM, detoffset=(900.0, -1500.0) if rot else (0.0, 0.0),
+                 detangles=(0.05, -0.08) if rot else (0.0, 0.0), nf=mode.startswith("nf"),
+                 scale="scale" in mode, secondorder="secondorder" in mode)
     at, _ = sc_ball(rng, 3000, 3e-4, 0.01)
     amp = rng.uniform(0.5, 1.0, len(at)) if "amp" in mode else None
     ref = oracle_mod.field_c(g, at, amp)
