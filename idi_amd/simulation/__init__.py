@@ -9,15 +9,18 @@ refers to it, as ``idi.simulation.auto`` refers to the CUDA backend when one is
 available (idi/simulation/__init__.py:9-31).  Unlike the reference there is no
 CPU fallback: on a machine without a ROCm GPU the calls raise RuntimeError.
 ``autograd`` holds the torch.autograd.Function with the hand-written adjoint.
-``cutime`` / ``time`` are the time-dependent simulation (decaying emitters), and
+``cutime`` / ``time`` are the time-dependent simulation (decaying emitters; ``autotime``
+refers to ``cutime``), and
 ``common`` holds photon sampling and detector charge sharing.
 """
 from . import common, cutime, hip, simobj, time  # noqa: F401
 from .common import chargesharing, randomphotons  # noqa: F401
 
 auto = hip
+autotime = cutime  # idi/simulation/__init__.py:14,30: the time-dependent backend
 
-__all__ = ["hip", "simobj", "auto", "autograd", "common", "cutime", "time", "randomphotons", "chargesharing"]
+__all__ = ["hip", "simobj", "auto", "autotime", "autograd", "common", "cutime", "time", "randomphotons",
+           "chargesharing"]
 
 
 def __getattr__(name):
