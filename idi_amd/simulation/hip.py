@@ -190,8 +190,8 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                  **kwargs):
     """Generator of simulated wavefields (cuda.py:16-138).
 
-    Two-deep pipeline on one stream: image i is copied back into pinned memory
-    (or cloned on the device), then the host draws the emitters of image i+1
+    Two-deep pipeline on one stream: image i is copied back into one of two pinned
+    buffers (or cloned on the device), then the host draws the emitters of image i+1
     while the GPU still computes image i, copies them through pinned memory and
     launches image i+1; image i is yielded as soon as its copy has landed, so the
     caller works on it while image i+1 computes.
@@ -224,6 +224,7 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
             a0 = _amplitudes(simobject, amplitudes)
             stages = [_Staging(simobject.N, a0 is not None, dev) for _ in range(2)]
             outs = [torch.empty((n0, n1), dtype=dtype, device=dev) for _ in range(2)]
+            hosts = [] if returngpu else [torch.empty((n0, n1), dtype=dtype, pin_memory=True) for _ in range(2)]
 
             source = _device_source(simobject) if device_rng else None
             draw_cache = {}
@@ -259,15 +260,14 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                 if returngpu:
                     ret = outs[slot].clone()
                 else:
-                    host = torch.empty((n0, n1), dtype=dtype, pin_memory=True)
-                    host.copy_(outs[slot], non_blocking=True)
+                    hosts[slot].copy_(outs[slot], non_blocking=True)
                     copied = torch.cuda.Event()
                     copied.record()
                 if count < maximg:
                     launch(1 - slot, _amplitudes(simobject, amplitudes))
                 if not returngpu:
                     copied.synchronize()
-                    ret = host.numpy()
+                    ret = hosts[slot].numpy().copy()  # pageable, as cupy's get() returns
                 yield ret
                 slot = 1 - slot
 
