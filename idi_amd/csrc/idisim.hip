@@ -804,7 +804,7 @@ int idisim_time_forward(idisim_handle* h, const double* emitters, const double* 
         return fail(IDISIM_E_INVALID, "idisim_time_forward: c and tau must be finite and > 0");
     DeviceGuard guard(h->device);
     const TimeLayout L = time_layout(h->g, n, h->precision == IDISIM_PREC_FP64, (size_t)1 << 30);
-    if (n > 0 && !time_fused_ok(n, h->time_path)) {
+    if (n > 0 && !time_fused_ok(n, L.fp64, h->time_path)) {
         int rc = ensure_scratch(h, L.bytes);
         if (rc) return rc;
     }

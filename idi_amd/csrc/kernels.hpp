@@ -92,7 +92,7 @@ TimeLayout time_layout(const Geom& g, int64_t n, bool fp64_keys, size_t budget);
 // path: 0 = by emitter count, 1 = one workgroup per pixel (n <= 10240), 2 = batched sort path
 hipError_t launch_time(const Geom& g, const TimeLayout& L, const double* emit, const double* times, int64_t n,
                        double c, double tau, int path, void* scratch, double* out, hipStream_t st);
-bool time_fused_ok(int64_t n, int path);
+bool time_fused_ok(int64_t n, bool fp64_keys, int path);
 
 // Choice of n of m sites without replacement, optionally weighted (choice.hip).
 size_t choice_workspace_bytes(int64_t m);

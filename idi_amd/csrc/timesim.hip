@@ -15,7 +15,8 @@
 // two ~detector-distance terms that the reference's formula has); arrival times are float32
 // keys on FP32-split handles (the reference's "mixed" precision) and float64 on FP64 handles.
 //
-// Up to 10240 emitters: k_time_fused, one workgroup per pixel, all in registers and LDS.
+// Up to 20480 emitters (float32 keys; 12288 with float64 keys): k_time_fused, one workgroup
+// of 256 or 512 lanes per pixel, all in registers and LDS.
 // Larger counts, per batch of pixels:
 //   k_time_prepare    one thread per (pixel, emitter): key t, index j, amplitude a (FP64);
 //   segmented radix sort of (t, j) per pixel (rocPRIM, stable);
@@ -33,8 +34,10 @@ namespace idisim {
 
 namespace {
 constexpr int kTimeK = 8;           // sorted arrivals per lane and chunk (batched path)
-constexpr int kFusedMaxItems = 40;  // arrivals per lane of the one-workgroup-per-pixel path
-constexpr int kFusedAutoMax = 10240;  // emitter counts that take that path by default
+// One workgroup per pixel: 256 lanes with up to 40 arrivals each, or 512 lanes with up to 40
+// (float32 keys) or 24 (float64 keys) arrivals each, as much as 160 KB of LDS holds.
+constexpr int kFusedMax32 = 512 * 40;
+constexpr int kFusedMax64 = 512 * 24;
 }
 
 namespace {
@@ -230,26 +233,26 @@ __global__ __launch_bounds__(256) void k_time_integrate(const Tt* __restrict__ k
 // states (scanned across lanes as above) and once more to step the field through the lane's
 // arrivals from its incoming state and sum the integral terms.  The key buffer, the sort's
 // storage and the sorted arrays share one LDS union; nothing goes through global memory.
-template <typename Tt, int ITEMS>
-__global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __restrict__ emit,
+template <typename Tt, int ITEMS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_time_fused(Geom g, const double* __restrict__ emit,
                                                     const double* __restrict__ times, int64_t n, double c,
                                                     double tau, double* __restrict__ out)
 {
-    using Sort = rocprim::block_radix_sort<Tt, 256, ITEMS, int>;
+    using Sort = rocprim::block_radix_sort<Tt, BLOCK, ITEMS, int>;
     constexpr int S = ITEMS + 1;  // padded stride of the sorted arrays
-    static_assert(256 * ITEMS <= 65536, "sorted indices are stored as 16 bits");
+    static_assert(BLOCK * ITEMS <= 65536, "sorted indices are stored as 16 bits");
     union Lds {
         typename Sort::storage_type sort;
-        Tt kbuf[256 * ITEMS];
+        Tt kbuf[BLOCK * ITEMS];
         struct {
-            Tt key[256 * S];
-            unsigned short idx[256 * S];
+            Tt key[BLOCK * S];
+            unsigned short idx[BLOCK * S];
         } sorted;
     };
     __shared__ Lds u;
-    __shared__ double sre[256], sim[256], st[256];
-    __shared__ int shas[256];
-    __shared__ double ssum[256];
+    __shared__ double sre[BLOCK], sim[BLOCK], st[BLOCK];
+    __shared__ int shas[BLOCK];
+    __shared__ double ssum[BLOCK];
     const int tid = threadIdx.x;
     const int64_t p = blockIdx.x;
     const double inv_tau = 1.0 / tau, two_over_tau = 2.0 / tau;
@@ -259,7 +262,7 @@ __global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __rest
 
     // arrival keys (emitters in order, coalesced reads)
 #pragma unroll 1
-    for (int j = tid; j < 256 * ITEMS; j += 256) {
+    for (int j = tid; j < BLOCK * ITEMS; j += BLOCK) {
         Tt key = (Tt)INFINITY;
         if (j < n) key = (Tt)arrival_time(g, r, R, emit + 4 * (int64_t)j, times[j], c);
         u.kbuf[j] = key;
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __rest
     shas[tid] = mine.has;
     __syncthreads();
     DecayState inc = mine;
-    for (int off = 1; off < 256; off <<= 1) {
+    for (int off = 1; off < BLOCK; off <<= 1) {
         DecayState left = {0.0, 0.0, 0.0, 0};
         if (tid >= off) left = {sre[tid - off], sim[tid - off], st[tid - off], shas[tid - off]};
         __syncthreads();
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(256) void k_time_fused(Geom g, const double* __rest
     const double acc = A + 2.0 * (xr * Br + xi * Bi) + (xr * xr + xi * xi) * C;
     ssum[tid] = acc;
     __syncthreads();
-    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    for (int s2 = BLOCK / 2; s2 > 0; s2 >>= 1) {
         if (tid < s2) ssum[tid] += ssum[tid + s2];
         __syncthreads();
     }
@@ -421,20 +424,34 @@ static hipError_t run_fused(const Geom& g, const double* emit, const double* tim
                             double* out, int64_t npix, hipStream_t st)
 {
     const dim3 grid((unsigned)npix);
-#define IDISIM_TIME_FUSED(I) \
-    hipLaunchKernelGGL((k_time_fused<Tt, I>), grid, dim3(256), 0, st, g, emit, times, n, c, tau, out)
+#define IDISIM_TIME_FUSED(I, B) \
+    hipLaunchKernelGGL((k_time_fused<Tt, I, B>), grid, dim3(B), 0, st, g, emit, times, n, c, tau, out)
     if (n <= 256 * 4)
-        IDISIM_TIME_FUSED(4);
+        IDISIM_TIME_FUSED(4, 256);
     else if (n <= 256 * 8)
-        IDISIM_TIME_FUSED(8);
+        IDISIM_TIME_FUSED(8, 256);
     else if (n <= 256 * 16)
-        IDISIM_TIME_FUSED(16);
+        IDISIM_TIME_FUSED(16, 256);
     else if (n <= 256 * 24)
-        IDISIM_TIME_FUSED(24);
+        IDISIM_TIME_FUSED(24, 256);
     else if (n <= 256 * 32)
-        IDISIM_TIME_FUSED(32);
-    else
-        IDISIM_TIME_FUSED(kFusedMaxItems);
+        IDISIM_TIME_FUSED(32, 256);
+    else if (n <= 256 * 40)
+        IDISIM_TIME_FUSED(40, 256);
+    else if (n <= 512 * 24)
+        IDISIM_TIME_FUSED(24, 512);
+    else {
+        if constexpr (sizeof(Tt) == 4) {  // float64 keys: 512 x 32 no longer fits the LDS
+            if (n <= 512 * 32)
+                IDISIM_TIME_FUSED(32, 512);
+            else if (n <= 512 * 40)
+                IDISIM_TIME_FUSED(40, 512);
+            else
+                return hipErrorInvalidValue;  // past the fused capacity (time_fused_ok)
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
 #undef IDISIM_TIME_FUSED
     return hipGetLastError();
 }
@@ -445,10 +462,9 @@ __global__ void k_time_zero(double* __restrict__ out, int64_t npix)
     if (i < npix) out[i] = 0.0;
 }
 
-bool time_fused_ok(int64_t n, int path)
+bool time_fused_ok(int64_t n, bool fp64_keys, int path)
 {
-    if (path == 2 || n > 256 * kFusedMaxItems) return false;
-    return path == 1 || n <= kFusedAutoMax;
+    return path != 2 && n <= (fp64_keys ? kFusedMax64 : kFusedMax32);
 }
 
 hipError_t launch_time(const Geom& g, const TimeLayout& L, const double* emit, const double* times, int64_t n,
@@ -459,7 +475,7 @@ hipError_t launch_time(const Geom& g, const TimeLayout& L, const double* emit, c
         hipLaunchKernelGGL(k_time_zero, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, out, npix);
         return hipGetLastError();
     }
-    if (time_fused_ok(n, path)) {
+    if (time_fused_ok(n, L.fp64, path)) {
         const int64_t npix = (int64_t)g.n0 * g.n1;
         if (L.fp64) return run_fused<double>(g, emit, times, n, c, tau, out, npix, st);
         return run_fused<float>(g, emit, times, n, c, tau, out, npix, st);

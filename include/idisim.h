@@ -81,9 +81,11 @@ extern "C" {
                                      revolution (default 5000 = 5e-6 rev)          */
 #define IDISIM_OPT_FALLBACK     7 /* 1 (default): FP32-split handles run the FP64
                                      kernel when no tile meets the bound; 0: error */
-#define IDISIM_OPT_TIME_PATH    8 /* time-dependent simulation: 0 (default) picks
-                                     by emitter count, 1 = one workgroup per pixel
-                                     (n <= 10240), 2 = batched sort path        */
+#define IDISIM_OPT_TIME_PATH    8 /* time-dependent simulation: 0 (default) and 1
+                                     take one workgroup per pixel when it fits
+                                     (n <= 20480 with float32 keys, 12288 with
+                                     float64 keys), else the batched sort path;
+                                     2 = always the batched sort path          */
 
 typedef struct idisim_handle idisim_handle;
 

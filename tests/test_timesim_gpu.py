@@ -36,10 +36,11 @@ def lib():
 
 
 @pytest.mark.parametrize("nf,scale", [(False, False), (True, False), (False, True), (True, True)])
-@pytest.mark.parametrize("n", [1, 2, 700, 5000])
+@pytest.mark.parametrize("n", [1, 2, 700, 5000, 12000, 20000])
 def test_time_forward_matches_restatement(lib, oracle_mod, nf, scale, n):
     """Both precisions, N across the 2048-arrival chunk boundary, arrivals interleaved by the
-    path differences (pulse comparable to r/c)."""
+    path differences (pulse comparable to r/c).  12000 and 20000 emitters take the 512-lane
+    workgroups (float64 keys at 20000: the batched sort path)."""
     import oracle
 
     rng = np.random.default_rng(n)
@@ -129,3 +130,25 @@ def test_time_paths_agree_and_empty_input(lib):
     assert np.all(empty == 0)
     with pytest.raises(lib.IdisimError):
         h.time_forward(d, t, C_UM_FS, -1.0)
+
+
+@pytest.mark.parametrize("n,precision", [(10500, 0), (12288, 0), (12289, 0), (16000, 1), (20480, 1), (20481, 1)])
+def test_time_paths_agree_across_fused_capacity(lib, n, precision):
+    """The 512-lane one-workgroup-per-pixel kernels against the batched sort path on both sides of
+    the fused capacity (12288 emitters with float64 keys, 20480 with float32 keys)."""
+    import torch
+
+    import oracle
+    from gpu_util import make_handle
+
+    rng = np.random.default_rng(n)
+    at = _atoms(rng, n, 0.02)
+    times = rng.normal(0, 5e-4, n)
+    g = oracle.Geometry(6, 5, 300.0, 1e5, K_UM, nf=True)
+    h = make_handle(lib, g, precision)
+    d, t = torch.from_numpy(at).cuda(), torch.from_numpy(times).cuda()
+    res = []
+    for path in (0, 2):
+        h.set_option(lib.OPT_TIME_PATH, path)
+        res.append(h.time_forward(d, t, C_UM_FS, 1e-3).cpu().numpy())
+    assert rel_l2(res[0], res[1]) <= 1e-12
