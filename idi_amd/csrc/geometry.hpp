@@ -164,11 +164,15 @@ struct WorkItem {
 // 4: MFMA kernel (kernels_mfma.hip), 16 x 16 pixels per workgroup.
 // 5, 6: mirror-pair VALU kernels (one or two 2 x 2 quads per lane; 32x32 and
 // 32x64 pixels per workgroup), which share one sincos per quad and emitter.
-constexpr int kNumTiles = 7;
+// 7: mirror pairs with a centre per pair (one quad per lane, 32x32 pixels): the
+// second centre is reached by a first-order rotation, so no remainder is dropped.
+constexpr int kNumTiles = 8;
 constexpr int kTileMfma = 4;
 constexpr int kTilePair1 = 5;
 constexpr int kTilePair2 = 6;
-__host__ __device__ constexpr bool tile_pair(int v) { return v == kTilePair1 || v == kTilePair2; }
+constexpr int kTilePairC = 7;
+__host__ __device__ constexpr bool tile_pair(int v) { return v == kTilePair1 || v == kTilePair2 || v == kTilePairC; }
+__host__ __device__ constexpr bool tile_pair_centred(int v) { return v == kTilePairC; }
 __host__ __device__ constexpr int pair_quads(int v) { return v == kTilePair2 ? 2 : 1; }
 __host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileMfma ? 2 : 1; }
 __host__ __device__ constexpr int tile_py(int v)

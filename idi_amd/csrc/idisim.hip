@@ -234,9 +234,17 @@ double plan_bound(const idisim_handle* h, int v, double rho)
         return std::numeric_limits<double>::infinity();  // mirror pairs need the fold
     }
     if (tile_pair(v)) {
-        e += h->pair_coef * rho * (1.0 + sig);  // dropped remainder e_k
-        e += kSinCosErrRev;                     // second sincos (delta_k)
-        e += 2.0 * u24;                         // e^{i tau_c} (cos, sin) delta_k products, as phase
+        const double rem = h->pair_coef * rho * (1.0 + sig);  // |e_k| = |(m_k - mc) . r'|
+        if (tile_pair_centred(v)) {
+            // pair 1's centre phasor to first order: |Delta| = 2 pi |(m_0 - m_1) . r'| <= 4 pi rem,
+            // error Delta^2 / 2 (rad), plus Delta's own FP32 rounding
+            const double delta = 4.0 * M_PI * rem;
+            e += (0.5 * delta * delta + 4.0 * u24 * delta) * kInv2Pi;
+        } else {
+            e += rem;  // dropped remainder e_k
+        }
+        e += kSinCosErrRev;  // second sincos (delta_k)
+        e += 2.0 * u24;      // e^{i tau_c} (cos, sin) delta_k products, as phase
     }
     return e;
 }
@@ -452,7 +460,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     } else {
         // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {kTilePair1, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTilePair1, kTilePairC, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -1051,7 +1059,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {kTilePair1, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTilePair1, kTilePairC, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;

@@ -213,7 +213,12 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
 // That is 6 transcendentals and 17 FMA per 4 pixels where split_tile spends 8
 // and 20.  The W term must be folded into A (no lam_p per pixel); sums are the
 // chunked Kahan sums of split_tile.
-template <int QY, bool HAS_AMP>
+// CENTRED: each pair gets its own centre, the pair's mean dq m_k, so that no
+// remainder is dropped.  tau_c is then pair 0's centre, and pair 1's phasor is
+// e^{i tau_c} e^{-i Delta} with Delta = 2 pi (m_0 - m_1) . r' (a few 1e-4 rad),
+// taken to first order as (1 - i Delta): 5 more FMA per quad, an error of
+// Delta^2 / 2 that the host planner bounds.
+template <int QY, bool HAS_AMP, bool CENTRED>
 __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restrict__ emit,
                                           const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0,
                                           int ty0, float4* s0, float2* s1, float (&acc)[QY][2][4],
@@ -227,7 +232,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
     tile_centre(g, tx0, ty0, TX, TY, ccx, ccy);
     const Frame c = pixel_frame(g, ccx, ccy);
 
-    float mc[QY][3], d[QY][2][3];
+    float mc[QY][3], d[QY][2][3], dv[QY][3];
 #pragma unroll
     for (int q = 0; q < QY; ++q) {
         double dq[2][2][3];
@@ -242,7 +247,10 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            mc[q][a] = (float)(0.25 * ((dq[0][0][a] + dq[1][1][a]) + (dq[1][0][a] + dq[0][1][a])));
+            const double m0 = 0.5 * (dq[0][0][a] + dq[1][1][a]);
+            const double m1 = 0.5 * (dq[1][0][a] + dq[0][1][a]);
+            mc[q][a] = (float)(CENTRED ? m0 : 0.5 * (m0 + m1));
+            dv[q][a] = (float)(kTwoPi * (m0 - m1));
             d[q][0][a] = (float)(0.5 * (dq[1][1][a] - dq[0][0][a]));
             d[q][1][a] = (float)(0.5 * (dq[0][1][a] - dq[1][0][a]));
         }
@@ -291,6 +299,14 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                     sc *= am;
                     cc *= am;
                 }
+                float pc[2] = {cc, cc}, ps[2] = {sc, sc};  // centre phasor of each pair
+                if (CENTRED) {
+                    float dl = dv[q][0] * r0.x;
+                    dl = fmaf(dv[q][1], r0.y, dl);
+                    dl = fmaf(dv[q][2], r0.z, dl);
+                    pc[1] = fmaf(dl, sc, cc);   // (cc + i sc)(1 - i dl)
+                    ps[1] = fmaf(-dl, cc, sc);
+                }
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     float dl = d[q][k][0] * r0.x;
@@ -298,10 +314,10 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                     dl = fmaf(d[q][k][2], r0.z, dl);
                     const float sd = __builtin_amdgcn_sinf(dl);
                     const float cd = __builtin_amdgcn_cosf(dl);
-                    su[q][k][0] = fmaf(cc, cd, su[q][k][0]);
-                    su[q][k][1] = fmaf(sc, cd, su[q][k][1]);
-                    su[q][k][2] = fmaf(cc, sd, su[q][k][2]);
-                    su[q][k][3] = fmaf(sc, sd, su[q][k][3]);
+                    su[q][k][0] = fmaf(pc[k], cd, su[q][k][0]);
+                    su[q][k][1] = fmaf(ps[k], cd, su[q][k][1]);
+                    su[q][k][2] = fmaf(pc[k], sd, su[q][k][2]);
+                    su[q][k][3] = fmaf(ps[k], sd, su[q][k][3]);
                 }
             }
         }
@@ -340,7 +356,7 @@ __device__ __forceinline__ void split_hi_lo(double v, float& hi, float& lo)
     lo = (float)((double)hi - v);
 }
 
-template <int QY, bool HAS_AMP>
+template <int QY, bool HAS_AMP, bool CENTRED>
 __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __restrict__ emit,
                                                      const double* __restrict__ amp, int64_t n, int tiles_y,
                                                      int64_t per_split, float4* __restrict__ partial, void* out,
@@ -355,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
     const int64_t e0 = (int64_t)split * per_split;
     const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
     float acc[QY][2][4], cmp[QY][2][4];
-    pair_tile<QY, HAS_AMP>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, acc, cmp);
+    pair_tile<QY, HAS_AMP, CENTRED>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, acc, cmp);
 
     const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
     const size_t npix = (size_t)g.n0 * g.n1;
@@ -385,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
 }
 
 // Persistent batched form of k_fwd_pair (the queue protocol of k_batched_split).
-template <int QY, bool HAS_AMP>
+template <int QY, bool HAS_AMP, bool CENTRED>
 __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* __restrict__ emit,
                                                          const double* __restrict__ amp,
                                                          const WorkItem* __restrict__ items, int nitems,
@@ -406,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* _
         const WorkItem w = items[it];
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         float acc[QY][2][4], cmp[QY][2][4];
-        pair_tile<QY, HAS_AMP>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, acc, cmp);
+        pair_tile<QY, HAS_AMP, CENTRED>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, acc, cmp);
         char* outs = static_cast<char*>(out) +
                      (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
 #pragma unroll
@@ -763,7 +779,7 @@ int occupancy_batched_split(int v, bool w, bool a, int s)
     return 1;
 }
 
-template <int QY, bool A>
+template <int QY, bool A, bool C>
 static hipError_t launch_pair_q(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
                                 float4* partial, void* out, int kind, hipStream_t st)
 {
@@ -771,34 +787,50 @@ static hipError_t launch_pair_q(const Geom& g, const double* emit, const double*
     const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
     const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
     dim3 grid(tiles_x * tiles_y, splits);
-    hipLaunchKernelGGL((k_fwd_pair<QY, A>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
+    hipLaunchKernelGGL((k_fwd_pair<QY, A, C>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
                        per > 0 ? per : (int64_t)kChunk, partial, out, kind);
     return hipGetLastError();
 }
 
-hipError_t launch_fwd_pair(int v, bool a, const Geom& g, const double* emit, const double* amp, int64_t n,
-                           int splits, float4* partial, void* out, int kind, hipStream_t st)
-{
-    if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;  // needs W folded into A
-    switch ((pair_quads(v) << 1) | (int)a) {
-        case 2: return launch_pair_q<1, false>(g, emit, amp, n, splits, partial, out, kind, st);
-        case 3: return launch_pair_q<1, true>(g, emit, amp, n, splits, partial, out, kind, st);
-        case 4: return launch_pair_q<2, false>(g, emit, amp, n, splits, partial, out, kind, st);
-        case 5: return launch_pair_q<2, true>(g, emit, amp, n, splits, partial, out, kind, st);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-template <int QY, bool A>
+template <int QY, bool A, bool C>
 static hipError_t launch_batched_pair_q(const Geom& g, const double* emit, const double* amp,
                                         const WorkItem* items, int nitems, int* counter, int nblocks, void* out,
                                         int kind, hipStream_t st)
 {
     const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL((k_batched_pair<QY, A>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp, items, nitems,
+    hipLaunchKernelGGL((k_batched_pair<QY, A, C>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp, items, nitems,
                        counter, tiles_y, out, kind);
     return hipGetLastError();
+}
+
+template <int QY, bool A, bool C>
+static int occupancy_pair_q()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_pair<QY, A, C>, kBlock, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+// (quads per lane, amplitudes, centred) -> instantiation; variant 7 has one quad per lane
+#define IDISIM_DISPATCH_PAIR(FN, ...)                                      \
+    switch ((pair_quads(v) << 2) | ((int)tile_pair_centred(v) << 1) | (int)a) { \
+        case 4: return FN<1, false, false>(__VA_ARGS__);                  \
+        case 5: return FN<1, true, false>(__VA_ARGS__);                   \
+        case 6: return FN<1, false, true>(__VA_ARGS__);                   \
+        case 7: return FN<1, true, true>(__VA_ARGS__);                    \
+        case 8: return FN<2, false, false>(__VA_ARGS__);                  \
+        case 9: return FN<2, true, false>(__VA_ARGS__);                   \
+        default: break;                                                   \
+    }
+
+hipError_t launch_fwd_pair(int v, bool a, const Geom& g, const double* emit, const double* amp, int64_t n,
+                           int splits, float4* partial, void* out, int kind, hipStream_t st)
+{
+    if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;  // needs W folded into A
+    IDISIM_DISPATCH_PAIR(launch_pair_q, g, emit, amp, n, splits, partial, out, kind, st)
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_batched_pair(int v, bool a, const Geom& g, const double* emit, const double* amp,
@@ -806,32 +838,14 @@ hipError_t launch_batched_pair(int v, bool a, const Geom& g, const double* emit,
                                hipStream_t st)
 {
     if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;
-    switch ((pair_quads(v) << 1) | (int)a) {
-        case 2: return launch_batched_pair_q<1, false>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
-        case 3: return launch_batched_pair_q<1, true>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
-        case 4: return launch_batched_pair_q<2, false>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
-        case 5: return launch_batched_pair_q<2, true>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-template <int QY, bool A>
-static int occupancy_pair_q()
-{
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_pair<QY, A>, kBlock, 0) != hipSuccess) return 1;
-    return nb > 0 ? nb : 1;
+    IDISIM_DISPATCH_PAIR(launch_batched_pair_q, g, emit, amp, items, nitems, counter, nblocks, out, kind, st)
+    return hipErrorInvalidValue;
 }
 
 int occupancy_batched_pair(int v, bool a)
 {
-    switch ((pair_quads(v) << 1) | (int)a) {
-        case 2: return occupancy_pair_q<1, false>();
-        case 3: return occupancy_pair_q<1, true>();
-        case 4: return occupancy_pair_q<2, false>();
-        case 5: return occupancy_pair_q<2, true>();
-        default: return 1;
-    }
+    IDISIM_DISPATCH_PAIR(occupancy_pair_q)
+    return 1;
 }
 
 hipError_t launch_finalize_split(const Geom& g, const float4* partial, int splits, void* out, int kind,

@@ -246,7 +246,8 @@ def test_autotune_cache_across_extents(lib, oracle_mod):
 
 @pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder", "nf_scale_amp", "nf_rot", "ff_rot"])
 def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
-    """Mirror-pair tiles (5: one 2 x 2 quad per lane, 6: two) on the pixel pitch and distance of
+    """Mirror-pair tiles (5: one 2 x 2 quad per lane, 6: two, 7: one with a centre per pair) on the
+    pixel pitch and distance of
     the 100k-site image (75 um at 0.2 m, sites of a 10 nm ball), on an odd detector so that quads
     reach past the last row and column; every emitter split and output kind; a rotated and
     offset detector."""
@@ -260,7 +261,7 @@ def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
     at, _ = sc_ball(rng, 3000, 3e-4, 0.01)
     amp = rng.uniform(0.5, 1.0, len(at)) if "amp" in mode else None
     ref = oracle_mod.field_c(g, at, amp)
-    for tile in (5, 6):
+    for tile in (5, 6, 7):
         for split in (1, 3):
             f, h = run(lib, g, at, amp, tile=tile, split=split)
             p = h.plan()
@@ -274,20 +275,26 @@ def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
 
 def test_pair_tiles_eligibility(lib, oracle_mod):
     """Pixels of 100 um at 0.1 m and a 30 nm ball: the pair remainder (~1.2e-5 rev) exceeds the
-    5e-6 rev tolerance, so the planner never picks a pair tile and a forced one reports its bound;
-    the per-term Kahan mode has no pair kernel."""
+    5e-6 rev tolerance, so the planner never picks tile 5 or 6 and a forced one reports its bound;
+    tile 7 (a centre per pair) drops no remainder and qualifies; the per-term Kahan mode has no
+    pair kernel."""
     from oracle import Geometry
 
     rng = np.random.default_rng(46)
     # the corner region of a 256^2 detector: offset by 96 pixels
     g = Geometry(64, 64, 100.0, 1e5, K_UM, detoffset=(9600.0, 9600.0), nf=True)
     at = atoms_ball(rng, 600, 0.03)
+    ref = oracle_mod.field_c(g, at)
     _, h = run(lib, g, at)
     assert h.plan()["tile"] not in (5, 6)
     f, h = run(lib, g, at, tile=5, split=1)
     assert h.plan()["phase_bound"] > 5e-6
-    with pytest.raises(lib.IdisimError):
-        run(lib, g, at, tile=5, split=1, summode=1)
+    f, h = run(lib, g, at, tile=7, split=2)
+    assert h.plan()["phase_bound"] <= 5e-6
+    assert rel_l2(f, ref) <= SPLIT_RTOL and rel_l2(intensity(f), intensity(ref)) <= SPLIT_RTOL
+    for tile in (5, 7):
+        with pytest.raises(lib.IdisimError):
+            run(lib, g, at, tile=tile, split=1, summode=1)
     # the same detector at 0.5 m: eligible, and within the bound of the oracle
     g2 = Geometry(64, 64, 100.0, 5e5, K_UM, detoffset=(9600.0, 9600.0), nf=True)
     f, h = run(lib, g2, at, tile=6, split=1)
