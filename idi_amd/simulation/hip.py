@@ -97,6 +97,25 @@ def _extent(atoms):
     return float(_np.sqrt(_np.max(_np.einsum("ij,ij->i", atoms[:, :3], atoms[:, :3])))) if len(atoms) else 0.0
 
 
+def _host_extent(simobject, pos, cache):
+    """Upper bound of |r_j| for the planner without a pass over the emitters where the object
+    fixes one: a crystal's sites all lie within its largest site norm (rotations keep norms),
+    a sphere's within its radius.  Otherwise the emitters are measured."""
+    from . import simobj as _so
+
+    if isinstance(simobject, _so.crystal):
+        allpos = simobject._allpos
+        key = (id(allpos), allpos.shape)
+        if cache.get("sites") != key:
+            cache["sites"] = key
+            a64 = _np.asarray(allpos, dtype=_np.float64)  # the sites may be float32; measure in FP64
+            cache["site_extent"] = float(_np.sqrt(_np.max(_np.einsum("ij,ij->i", a64, a64)))) if len(a64) else 0.0
+        return cache["site_extent"] * (1.0 + 1e-12)
+    if type(simobject) is _so.sphere:
+        return float(simobject.r) * (1.0 + 1e-12)
+    return _extent(pos)
+
+
 class _Staging:
     """Pinned host buffer + device buffer for one image's emitters."""
 
@@ -109,21 +128,32 @@ class _Staging:
         self.da = torch.empty((n,), dtype=torch.float64, device=device) if with_amp else None
         self.ready = None  # event after the H2D copy; the host buffer may be rewritten once it has passed
 
-    def upload(self, atoms, amp):
-        """Copy one image's emitters (None: they are drawn on the device) and amplitudes
-        (None: none) through the pinned buffers, asynchronously on the current stream."""
+    def upload(self, atoms, amp, stream, after=None):
+        """Copy one image's emitters (an (N, 4) array, a (positions, phases) pair as get2()
+        returns, or None: they are drawn on the device) and amplitudes (None: none) through
+        the pinned buffers, asynchronously on `stream`, once the event `after` (the last
+        kernel that read the device buffers) has passed.  Returns the event that marks the
+        copies done."""
         import torch
 
         if self.ready is not None:
-            self.ready.synchronize()
-        if atoms is not None:
-            self.h.numpy()[...] = atoms
-            self.d.copy_(self.h, non_blocking=True)
-        if amp is not None:
-            self.ha.numpy()[...] = amp
-            self.da.copy_(self.ha, non_blocking=True)
-        self.ready = torch.cuda.Event()
-        self.ready.record()
+            self.ready.synchronize()  # the previous copy out of the pinned buffers has finished
+        with torch.cuda.stream(stream):
+            if after is not None:
+                stream.wait_event(after)
+            if atoms is not None:
+                hv = self.h.numpy()
+                if isinstance(atoms, tuple):  # get2(): written straight into the pinned buffer
+                    hv[:, :3], hv[:, 3:] = atoms
+                else:
+                    hv[...] = atoms
+                self.d.copy_(self.h, non_blocking=True)
+            if amp is not None:
+                self.ha.numpy()[...] = amp
+                self.da.copy_(self.ha, non_blocking=True)
+            self.ready = torch.cuda.Event()
+            self.ready.record(stream)
+        return self.ready
 
 
 def _device_source(simobject):
@@ -190,11 +220,12 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                  **kwargs):
     """Generator of simulated wavefields (cuda.py:16-138).
 
-    Two-deep pipeline on one stream: image i is copied back into one of two pinned
-    buffers (or cloned on the device), then the host draws the emitters of image i+1
-    while the GPU still computes image i, copies them through pinned memory and
-    launches image i+1; image i is yielded as soon as its copy has landed, so the
-    caller works on it while image i+1 computes.
+    Two-deep pipeline: kernels run on the current stream, copies on a second stream
+    ordered with events.  While the GPU computes image i, the host draws the
+    emitters of image i+1 and they are copied in through pinned memory; image i+1
+    is launched behind image i, image i is copied out into one of two pinned
+    buffers (or cloned on the device) and yielded as soon as that copy has landed,
+    so the caller works on it while image i+1 computes.
 
     device_rng=True (extension): ``simobj.sphere``, ``simobj.multisphere`` (ball
     centres and counts still drawn on the host), ``simobj.gauss`` / ``gnorm``
@@ -221,25 +252,31 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
         if maximg <= 0:
             return
         with torch.cuda.device(dev):
+            cs = torch.cuda.current_stream()
+            xs = torch.cuda.Stream()  # host <-> device copies (default priority)
             a0 = _amplitudes(simobject, amplitudes)
             stages = [_Staging(simobject.N, a0 is not None, dev) for _ in range(2)]
             outs = [torch.empty((n0, n1), dtype=dtype, device=dev) for _ in range(2)]
             hosts = [] if returngpu else [torch.empty((n0, n1), dtype=dtype, pin_memory=True) for _ in range(2)]
+            for t in [st.d for st in stages] + [st.da for st in stages if st.da is not None] + outs:
+                t.record_stream(xs)  # device buffers the copy stream uses
+            fwd_done = [None, None]  # kernel of the image in slot k finished reading stages[k], writing outs[k]
+            copied = [None, None]  # outs[k] copied out to hosts[k]
 
             source = _device_source(simobject) if device_rng else None
             draw_cache = {}
             if source == "phases":
                 fixed = simobject.get()
                 for st in stages:
-                    st.upload(fixed, a0)
+                    cs.wait_event(st.upload(fixed, a0, xs))
                 fixed_extent = _extent(fixed)
 
             def launch(slot, amp):
                 st = stages[slot]
                 if source is None:
-                    atoms = simobject.get()
-                    st.upload(atoms, amp)
-                    ext = _extent(atoms)
+                    pos, phase = simobject.get2()
+                    ext = _host_extent(simobject, pos, draw_cache)
+                    cs.wait_event(st.upload((pos, phase), amp, xs, fwd_done[slot]))
                 else:
                     seed = int(simobject.rng.integers(2**63))
                     if source == "phases":
@@ -248,28 +285,36 @@ def simulate_gen(simobject, Ndet, pixelsize, detz, k=None, settings="double", ma
                     else:
                         ext = _device_draw(source, simobject, st.d, seed, draw_cache)
                     if amp is not None:
-                        st.upload(None, amp)
+                        cs.wait_event(st.upload(None, amp, xs, fwd_done[slot]))
+                if copied[slot] is not None:
+                    cs.wait_event(copied[slot])  # outs[slot] still held the image before
                 h.forward(st.d, st.da if amp is not None else None, outs[slot], kind, ext)
+                fwd_done[slot] = torch.cuda.Event()
+                fwd_done[slot].record(cs)
 
-            launch(0, a0)
-            slot = 0
-            while count < maximg:
-                count += 1
-                # image `count` is copied out before the next image is launched: the
-                # consumer gets it while the GPU computes the next one
-                if returngpu:
-                    ret = outs[slot].clone()
-                else:
-                    hosts[slot].copy_(outs[slot], non_blocking=True)
-                    copied = torch.cuda.Event()
-                    copied.record()
-                if count < maximg:
-                    launch(1 - slot, _amplitudes(simobject, amplitudes))
-                if not returngpu:
-                    copied.synchronize()
-                    ret = hosts[slot].numpy().copy()  # pageable, as cupy's get() returns
-                yield ret
-                slot = 1 - slot
+            try:
+                launch(0, a0)
+                slot = 0
+                while count < maximg:
+                    count += 1
+                    if returngpu:
+                        ret = outs[slot].clone()  # on the compute stream, ahead of the next kernels
+                    if count < maximg:
+                        # the upload of image count+1 is queued on the copy stream ahead of the
+                        # copy-out of image count, so it overlaps the kernel of image count
+                        launch(1 - slot, _amplitudes(simobject, amplitudes))
+                    if not returngpu:
+                        with torch.cuda.stream(xs):
+                            xs.wait_event(fwd_done[slot])
+                            hosts[slot].copy_(outs[slot], non_blocking=True)
+                            copied[slot] = torch.cuda.Event()
+                            copied[slot].record(xs)
+                        copied[slot].synchronize()
+                        ret = hosts[slot].numpy().copy()  # pageable, as cupy's get() returns
+                    yield ret
+                    slot = 1 - slot
+            finally:
+                xs.synchronize()
 
     return _gen()
 

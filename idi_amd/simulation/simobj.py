@@ -330,7 +330,8 @@ class crystal(simobj):
             new = self.rng.choice(len(self._allpos), min(missing, len(self._allpos)), replace=False, p=p)
             idx.append(new)
             missing -= len(new)
-        self._pos = _np.asarray(pos[_np.concatenate(idx)], dtype=_np.float64)
+        sel = idx[0] if len(idx) == 1 else _np.concatenate(idx)
+        self._pos = _np.take(pos, sel, axis=0).astype(_np.float64, copy=False)  # take: faster than fancy indexing
 
     def __setattr__(self, name, value):
         if name in self._resetproperties:

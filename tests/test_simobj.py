@@ -178,3 +178,25 @@ def test_abs2_and_randomphotons():
     assert ph.shape == p.shape
     assert abs(ph.sum(axis=(1, 2)).mean() / 1e4 - 1) < 0.05
     assert randomphotons(p, np.inf) is p
+
+
+def test_host_extent_bounds_the_emitters():
+    """The planner's extent for host-drawn images: a crystal's largest site norm (float32 sites
+    measured in FP64, unchanged by random orientations) and a sphere's radius bound every draw."""
+    from idi_amd.simulation import hip, simobj
+
+    cache = {}
+    for seed in range(4):
+        obj = simobj.sc(8000, 1000, 3e-4, repeats=[12, 12, 12])
+        obj.rng = np.random.default_rng(seed)
+        obj.rndOrientation = bool(seed % 2)
+        pos, _ = obj.get2()
+        assert hip._host_extent(obj, pos, cache) >= np.linalg.norm(pos, axis=1).max()
+    ball = simobj.sphere(8000, 2000, 0.05)
+    ball.rng = np.random.default_rng(1)
+    pos, _ = ball.get2()
+    assert hip._host_extent(ball, pos, cache) >= np.linalg.norm(pos, axis=1).max()
+    gauss = simobj.gauss(8000, 500, 0.02)
+    gauss.rng = np.random.default_rng(2)
+    pos, _ = gauss.get2()
+    assert hip._host_extent(gauss, pos, cache) == pytest.approx(np.linalg.norm(pos, axis=1).max(), rel=1e-15)
