@@ -109,11 +109,12 @@ class StepTimer:
 def cpu_baseline_rows(oracle, geom, atoms, amp, budget_s, nthreads):
     """Time the FP64 oracle on evenly spaced rows until ~budget_s of CPU work; returns rows, field, rate."""
     n0 = geom.n0
+    probe = np.unique(np.linspace(0, n0 - 1, 4).round().astype(np.int64))
+    oracle.field_c(geom, atoms, amp, rows=probe[:1], nthreads=nthreads)  # thread start-up, page-in
     t0 = time.perf_counter()
-    first = [n0 // 2]
-    f0 = oracle.field_c(geom, atoms, amp, rows=first, nthreads=nthreads)
+    f0 = oracle.field_c(geom, atoms, amp, rows=probe, nthreads=nthreads)
     dt = time.perf_counter() - t0
-    per_row = max(dt, 1e-3)
+    per_row = max(dt / len(probe), 1e-3)
     nrows = int(min(n0, max(4, budget_s / per_row)))
     rows = np.unique(np.linspace(0, n0 - 1, nrows).round().astype(np.int64))
     t0 = time.perf_counter()
@@ -506,7 +507,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep", "e2e"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the bench, run one elementwise copy of MIB MiB (reads and writes MIB MiB) "
