@@ -300,3 +300,28 @@ def test_pair_tiles_eligibility(lib, oracle_mod):
     f, h = run(lib, g2, at, tile=6, split=1)
     assert h.plan()["phase_bound"] <= 5e-6
     assert rel_l2(f, oracle_mod.field_c(g2, at)) <= SPLIT_RTOL
+
+
+def test_pair_tiles_tiny_detector_and_shots(lib, oracle_mod):
+    """Pair tiles where every quad reaches past the detector (5 x 3 and 1 x 1 pixels) and with one or
+    two emitters, single shot and batched."""
+    import torch
+
+    from oracle import Geometry
+
+    rng = np.random.default_rng(47)
+    for n0, n1 in ((5, 3), (1, 1)):
+        g = Geometry(n0, n1, 75.0, 2e5, K_UM, nf=True)
+        for n in (1, 2, 300):
+            at, _ = sc_ball(rng, n, 3e-4, 0.01)
+            ref = oracle_mod.field_c(g, at)
+            for tile in (5, 6, 7):
+                f, h = run(lib, g, at, tile=tile, split=1)
+                assert f.shape == (n0, n1)
+                assert rel_l2(f, ref) <= SPLIT_RTOL, (n0, n1, n, tile)
+            h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+            h.set_option(lib.OPT_TILE, 7)
+            offsets = np.array([0, n, n, 2 * n], dtype=np.int64)
+            out = h.forward_batched(torch.from_numpy(np.concatenate([at, at])).cuda(), offsets).cpu().numpy()
+            assert np.count_nonzero(out[1]) == 0
+            assert rel_l2(out[0], ref) <= SPLIT_RTOL and rel_l2(out[2], ref) <= SPLIT_RTOL
