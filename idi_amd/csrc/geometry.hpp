@@ -166,18 +166,24 @@ struct WorkItem {
 // 32x64 pixels per workgroup), which share one sincos per quad and emitter.
 // 7: mirror pairs with a centre per pair (one quad per lane, 32x32 pixels): the
 // second centre is reached by a first-order rotation, so no remainder is dropped.
-constexpr int kNumTiles = 8;
+// 8: four mirror pairs about a 2 x 4 block per lane (32x64 pixels), a centre per
+// pair, one sincos of a centre per block.
+constexpr int kNumTiles = 9;
 constexpr int kTileMfma = 4;
 constexpr int kTilePair1 = 5;
 constexpr int kTilePair2 = 6;
 constexpr int kTilePairC = 7;
-__host__ __device__ constexpr bool tile_pair(int v) { return v == kTilePair1 || v == kTilePair2 || v == kTilePairC; }
+constexpr int kTileBlock = 8;
+__host__ __device__ constexpr bool tile_pair(int v)
+{
+    return v == kTilePair1 || v == kTilePair2 || v == kTilePairC || v == kTileBlock;
+}
 __host__ __device__ constexpr bool tile_pair_centred(int v) { return v == kTilePairC; }
 __host__ __device__ constexpr int pair_quads(int v) { return v == kTilePair2 ? 2 : 1; }
 __host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileMfma ? 2 : 1; }
 __host__ __device__ constexpr int tile_py(int v)
 {
-    return v == 0 || v == kTileMfma ? 1 : (v == 3 || v == kTilePair2 ? 4 : 2);
+    return v == 0 || v == kTileMfma ? 1 : (v == 3 || v == kTilePair2 || v == kTileBlock ? 4 : 2);
 }
 __host__ __device__ constexpr int tile_rows(int v) { return 16 * tile_px(v); }
 __host__ __device__ constexpr int tile_cols(int v) { return 16 * tile_py(v); }

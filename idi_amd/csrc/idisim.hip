@@ -88,6 +88,7 @@ struct idisim_handle {
     double dR_max[kNumTiles] = {};
     double Rmin = 0.0, Rmax = 0.0;
     double pair_coef = 0.0;   // mirror-pair remainder per unit emitter extent (rev per length)
+    double block_coef = 0.0;  // bound of |m_k - m_1| of the 2 x 4 block pairs per unit extent
     double rho = 0.0;   // emitter extent of the current call
     // device scratch
     void* scratch = nullptr;
@@ -175,6 +176,17 @@ void planner_tables(idisim_handle* h)
     const double sm = std::min(1.0, emax / h->Rmin);
     h->pair_coef = 1.05 * g.krev * g.pix * g.pix * (std::sqrt(2.0) * sm + 3.0 * sm * sm) /
                    (4.0 * h->Rmin * h->Rmin);
+    // 2 x 4 block pairs: m_k - m_1 = (D2f[o_k] - D2f[o_1]) / 2 + O(o^4), where
+    // |D2f[o]| <= krev pix^2 |o|^2 (1 + (2 + sqrt 2) s + 6 s^2) / R^2 from
+    // d_a d_a rh = (3 alpha^2 rh - rh - 2 alpha a) / R^2 and the mixed derivative
+    // above; |o_k|^2 <= 5/2, |o_1|^2 = 1/2.  Blocks reach three columns past the
+    // last one.
+    double ebl = emax;
+    for (double fy : {-1.0, (double)g.n1 + 2.0})
+        ebl = std::max(ebl, std::fabs((fy - g.half1) * g.pix + g.off1));
+    const double sb = std::min(1.0, ebl / h->Rmin);
+    h->block_coef = 1.05 * 1.5 * (1.0 + (2.0 + std::sqrt(2.0)) * sb + 6.0 * sb * sb) * g.krev * g.pix * g.pix /
+                    (h->Rmin * h->Rmin);
 }
 
 // Error (revolutions) of folding W_cj into A_cj, i.e. of replacing lam_p W_cj
@@ -235,7 +247,11 @@ double plan_bound(const idisim_handle* h, int v, double rho)
     }
     if (tile_pair(v)) {
         const double rem = h->pair_coef * rho * (1.0 + sig);  // |e_k| = |(m_k - mc) . r'|
-        if (tile_pair_centred(v)) {
+        if (v == kTileBlock) {
+            // the three rotated centres, first order: |Delta| <= 2 pi block_coef rho (1 + sigma)
+            const double delta = 2.0 * M_PI * h->block_coef * rho * (1.0 + sig);
+            e += (0.5 * delta * delta + 4.0 * u24 * delta) * kInv2Pi;
+        } else if (tile_pair_centred(v)) {
             // pair 1's centre phasor to first order: |Delta| = 2 pi |(m_0 - m_1) . r'| <= 4 pi rem,
             // error Delta^2 / 2 (rad), plus Delta's own FP32 rounding
             const double delta = 4.0 * M_PI * rem;
@@ -460,7 +476,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     } else {
         // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {kTilePair1, kTilePairC, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -1059,7 +1075,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {kTilePair1, kTilePairC, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;

@@ -330,6 +330,136 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP32-split tile with four mirror pairs about a 2 x 4 block (tile variant 8; the
+// 16 x 16 lanes cover 32 x 64 pixels).  Pair k joins pixel (0, k) (first) and
+// (1, 3 - k) (second) of the lane's block, both at offset o_k from the block
+// centre.  Every pair has its own centre m_k (its mean dq), so nothing is
+// dropped; one sincos at pair 1's centre (|o_1| smallest) serves all four, the
+// others reached by the first-order rotation e^{-i Delta_k} ~ (1 - i Delta_k),
+// Delta_k = 2 pi (m_1 - m_k) . r'.  All pairs share the block centre, so the
+// m_k differ only by second-order terms and Delta_k stays a few 1e-3 rad at
+// most (the host planner bounds Delta_k^2 / 2).  Per 8 pixels: 10
+// transcendentals and 46 FMA.  The W term must be folded into A.
+template <bool HAS_AMP>
+__device__ __forceinline__ void block_tile(const Geom& g, const double* __restrict__ emit,
+                                           const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0, int ty0,
+                                           float4* s0, float2* s1, float (&acc)[4][4], float (&cmp)[4][4])
+{
+    constexpr int TX = 32, TY = 64;
+    const int tid = threadIdx.x;
+    const int lx = tid >> 4, ly = tid & 15;
+
+    double ccx, ccy;
+    tile_centre(g, tx0, ty0, TX, TY, ccx, ccy);
+    const Frame c = pixel_frame(g, ccx, ccy);
+
+    float mc[3], d[4][3], dv[4][3];
+    {
+        double dq[2][4][3];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const Frame f = pixel_frame(g, (double)(tx0 + 2 * lx + i), (double)(ty0 + 4 * ly + j));
+#pragma unroll
+                for (int a = 0; a < 3; ++a) dq[i][j][a] = -g.krev * (f.u[a] - c.u[a]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double m[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                m[k] = 0.5 * (dq[0][k][a] + dq[1][3 - k][a]);
+                d[k][a] = (float)(0.5 * (dq[1][3 - k][a] - dq[0][k][a]));
+            }
+            mc[a] = (float)m[1];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) dv[k][a] = (float)(kTwoPi * (m[1] - m[k]));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            acc[k][n] = 0.f;
+            cmp[k][n] = 0.f;
+        }
+    }
+
+    for (int64_t base = e0; base < e1; base += kChunk) {
+        const int cnt = (e1 - base) < (int64_t)kChunk ? (int)(e1 - base) : kChunk;
+        __syncthreads();  // the previous chunk has been consumed by every lane
+        if (tid < cnt) {
+            const int64_t e = base + tid;
+            const double* a4 = emit + 4 * e;
+            const double ea = (HAS_AMP && amp != nullptr) ? amp[e] : 1.0;
+            const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], ea);
+            s0[tid] = make_float4(r.x, r.y, r.z, r.A);
+            s1[tid] = make_float2(r.W, r.amp);
+        }
+        __syncthreads();
+
+        float su[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) su[k][n] = 0.f;
+#pragma unroll 2
+        for (int e = 0; e < cnt; ++e) {
+            const float4 r0 = s0[e];
+            float t = fmaf(mc[0], r0.x, r0.w);
+            t = fmaf(mc[1], r0.y, t);
+            t = fmaf(mc[2], r0.z, t);
+            float sc = __builtin_amdgcn_sinf(t);
+            float cc = __builtin_amdgcn_cosf(t);
+            if (HAS_AMP) {
+                const float am = s1[e].y;
+                sc *= am;
+                cc *= am;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float pc = cc, ps = sc;
+                if (k != 1) {
+                    float dl = dv[k][0] * r0.x;
+                    dl = fmaf(dv[k][1], r0.y, dl);
+                    dl = fmaf(dv[k][2], r0.z, dl);
+                    pc = fmaf(dl, sc, cc);  // (cc + i sc)(1 - i dl)
+                    ps = fmaf(-dl, cc, sc);
+                }
+                float dl = d[k][0] * r0.x;
+                dl = fmaf(d[k][1], r0.y, dl);
+                dl = fmaf(d[k][2], r0.z, dl);
+                const float sd = __builtin_amdgcn_sinf(dl);
+                const float cd = __builtin_amdgcn_cosf(dl);
+                su[k][0] = fmaf(pc, cd, su[k][0]);
+                su[k][1] = fmaf(ps, cd, su[k][1]);
+                su[k][2] = fmaf(pc, sd, su[k][2]);
+                su[k][3] = fmaf(ps, sd, su[k][3]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) kahan_add(acc[k][n], cmp[k][n], su[k][n]);
+    }
+}
+
+// Pixel (i, j) of a lane's 2 x 4 block: pair k = j (first member, i = 0) or 3 - j (second, i = 1).
+__device__ __forceinline__ void block_pixel(const float (&acc)[4][4], const float (&cmp)[4][4], int i, int j,
+                                            double& re, double& im)
+{
+    const int k = (i == 0) ? j : 3 - j;
+    const double ur = (double)acc[k][0] - (double)cmp[k][0];
+    const double ui = (double)acc[k][1] - (double)cmp[k][1];
+    const double vr = (double)acc[k][2] - (double)cmp[k][2];
+    const double vi = (double)acc[k][3] - (double)cmp[k][3];
+    re = (i == 1) ? ur - vi : ur + vi;
+    im = (i == 1) ? ui + vr : ui - vr;
+}
+
 // The four pixels of quad q of a lane: pixel (i, j) of the quad, its field in FP64
 // from the compensated U and V sums of its pair.
 template <int QY>
@@ -394,6 +524,86 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
                     } else {
                         store_split(g, x, y, re, im, out, kind, idx);
                     }
+                }
+            }
+        }
+    }
+}
+
+template <bool HAS_AMP>
+__global__ __launch_bounds__(kBlock) void k_fwd_block(Geom g, const double* __restrict__ emit,
+                                                      const double* __restrict__ amp, int64_t n, int tiles_y,
+                                                      int64_t per_split, float4* __restrict__ partial, void* out,
+                                                      int kind)
+{
+    constexpr int TX = 32, TY = 64;
+    __shared__ float4 s0[kChunk];
+    __shared__ float2 s1[kChunk];
+    const int tile = blockIdx.x;
+    const int split = blockIdx.y;
+    const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
+    const int64_t e0 = (int64_t)split * per_split;
+    const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
+    float acc[4][4], cmp[4][4];
+    block_tile<HAS_AMP>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, acc, cmp);
+
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int x = tx0 + 2 * lx + i, y = ty0 + 4 * ly + j;
+            if (x < g.n0 && y < g.n1) {
+                double re, im;
+                block_pixel(acc, cmp, i, j, re, im);
+                const size_t idx = (size_t)x * g.n1 + y;
+                if (partial != nullptr) {
+                    float rh, rl, ih, il;
+                    split_hi_lo(re, rh, rl);
+                    split_hi_lo(im, ih, il);
+                    partial[(size_t)split * npix + idx] = make_float4(rh, ih, rl, il);
+                } else {
+                    store_split(g, x, y, re, im, out, kind, idx);
+                }
+            }
+        }
+    }
+}
+
+template <bool HAS_AMP>
+__global__ __launch_bounds__(kBlock) void k_batched_block(Geom g, const double* __restrict__ emit,
+                                                          const double* __restrict__ amp,
+                                                          const WorkItem* __restrict__ items, int nitems,
+                                                          int* __restrict__ counter, int tiles_y, void* out, int kind)
+{
+    constexpr int TX = 32, TY = 64;
+    __shared__ float4 s0[kChunk];
+    __shared__ float2 s1[kChunk];
+    __shared__ int s_item;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const size_t npix = (size_t)g.n0 * g.n1;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nitems) break;
+        const WorkItem w = items[it];
+        const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
+        float acc[4][4], cmp[4][4];
+        block_tile<HAS_AMP>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, acc, cmp);
+        char* outs = static_cast<char*>(out) +
+                     (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int x = tx0 + 2 * lx + i, y = ty0 + 4 * ly + j;
+                if (x < g.n0 && y < g.n1) {
+                    double re, im;
+                    block_pixel(acc, cmp, i, j, re, im);
+                    store_split(g, x, y, re, im, outs, kind, (size_t)x * g.n1 + y);
                 }
             }
         }
@@ -825,10 +1035,46 @@ static int occupancy_pair_q()
         default: break;                                                   \
     }
 
+template <bool A>
+static hipError_t launch_block_a(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
+                                 float4* partial, void* out, int kind, hipStream_t st)
+{
+    const int tiles_x = (g.n0 + 31) / 32;
+    const int tiles_y = (g.n1 + 63) / 64;
+    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    dim3 grid(tiles_x * tiles_y, splits);
+    hipLaunchKernelGGL((k_fwd_block<A>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
+                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+    return hipGetLastError();
+}
+
+template <bool A>
+static hipError_t launch_batched_block_a(const Geom& g, const double* emit, const double* amp,
+                                         const WorkItem* items, int nitems, int* counter, int nblocks, void* out,
+                                         int kind, hipStream_t st)
+{
+    const int tiles_y = (g.n1 + 63) / 64;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL((k_batched_block<A>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp, items, nitems,
+                       counter, tiles_y, out, kind);
+    return hipGetLastError();
+}
+
+template <bool A>
+static int occupancy_block_a()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_block<A>, kBlock, 0) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
 hipError_t launch_fwd_pair(int v, bool a, const Geom& g, const double* emit, const double* amp, int64_t n,
                            int splits, float4* partial, void* out, int kind, hipStream_t st)
 {
     if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;  // needs W folded into A
+    if (v == kTileBlock)
+        return a ? launch_block_a<true>(g, emit, amp, n, splits, partial, out, kind, st)
+                 : launch_block_a<false>(g, emit, amp, n, splits, partial, out, kind, st);
     IDISIM_DISPATCH_PAIR(launch_pair_q, g, emit, amp, n, splits, partial, out, kind, st)
     return hipErrorInvalidValue;
 }
@@ -838,12 +1084,16 @@ hipError_t launch_batched_pair(int v, bool a, const Geom& g, const double* emit,
                                hipStream_t st)
 {
     if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;
+    if (v == kTileBlock)
+        return a ? launch_batched_block_a<true>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st)
+                 : launch_batched_block_a<false>(g, emit, amp, items, nitems, counter, nblocks, out, kind, st);
     IDISIM_DISPATCH_PAIR(launch_batched_pair_q, g, emit, amp, items, nitems, counter, nblocks, out, kind, st)
     return hipErrorInvalidValue;
 }
 
 int occupancy_batched_pair(int v, bool a)
 {
+    if (v == kTileBlock) return a ? occupancy_block_a<true>() : occupancy_block_a<false>();
     IDISIM_DISPATCH_PAIR(occupancy_pair_q)
     return 1;
 }

@@ -73,8 +73,9 @@ extern "C" {
                                      (16x16 .. 32x64 px), 4 MFMA (FP32-split only),
                                      5, 6 mirror pairs (FP32-split only, 32x32 and
                                      32x64 px), 7 mirror pairs with a centre per
-                                     pair (FP32-split only, 32x32 px); FP64
-                                     handles take 0..3                           */
+                                     pair (FP32-split only, 32x32 px), 8 four
+                                     mirror pairs per 2x4 block (FP32-split only,
+                                     32x64 px); FP64 handles take 0..3           */
 #define IDISIM_OPT_SPLIT        3 /* force emitter split count (0 = planner)        */
 #define IDISIM_OPT_SUMMODE      4 /* 0 (default): per-chunk partials merged with
                                      Kahan; 1: Kahan on every term (cpu.py:97-105) */

@@ -94,3 +94,40 @@ def test_pair_remainder_isotropic_part_cancels():
     u = _unit_dirs(g, np.arange(2.0), np.arange(2.0))
     row_pair = 0.5 * (u[:, 0, 0] + u[:, 0, 1]) - 0.25 * u.sum(axis=(1, 2))
     assert krev * np.linalg.norm(row_pair) > 0.1 * krev * g.pixelsize**2 / g.detz**2
+
+
+def _block_bound_coef(g):
+    """1.05 * 3/2 * (1 + (2 + sqrt 2) s + 6 s^2) krev pix^2 / Rmin^2, with s over the detector plus
+    the three columns a 2 x 4 block reaches past the last one."""
+    ex = max(abs((fx - g.n0 / 2) * g.pixelsize + g.detoffset[0]) for fx in (-1.0, float(g.n0)))
+    ey = max(abs((fy - g.n1 / 2) * g.pixelsize + g.detoffset[1]) for fy in (-1.0, float(g.n1 + 2)))
+    xs = np.arange(g.n0, dtype=np.float64)
+    ys = np.arange(g.n1, dtype=np.float64)
+    detx = (xs[:, None] - g.n0 / 2) * g.pixelsize + g.detoffset[0]
+    dety = (ys[None, :] - g.n1 / 2) * g.pixelsize + g.detoffset[1]
+    rmin = float(np.sqrt(detx**2 + dety**2 + g.detz**2).min())
+    s = min(1.0, max(ex, ey) / rmin)
+    return 1.05 * 1.5 * (1 + (2 + np.sqrt(2)) * s + 6 * s * s) * g.k / (2 * np.pi) * g.pixelsize**2 / rmin**2
+
+
+def _block_true_coef(g):
+    """max over 2 x 4 blocks and pairs k of krev |m_k - m_1|, pair k = pixels (0, k) and (1, 3 - k)."""
+    n0q, n1q = (g.n0 + 1) // 2, (g.n1 + 3) // 4
+    u = _unit_dirs(g, np.arange(2 * n0q, dtype=np.float64), np.arange(4 * n1q, dtype=np.float64))
+    m = [0.5 * (u[:, 0::2, k::4] + u[:, 1::2, (3 - k)::4]) for k in range(4)]
+    return g.k / (2 * np.pi) * max(float(np.linalg.norm(m[k] - m[1], axis=0).max()) for k in range(4))
+
+
+@pytest.mark.parametrize("args,slack", CASES)
+def test_block_pair_centre_bound(args, slack):
+    """The bound of the spread of the four pair centres of a 2 x 4 block (the rotation angles of
+    tile variant 8), against the exact spread of every block; the slack allowed is the pair
+    remainder's plus a factor 1.5 (the isotropic curvature does not cancel between |o|^2 = 1/2 and
+    5/2, and the bound takes |o_k|^2 + |o_1|^2 as a whole)."""
+    from oracle import Geometry
+
+    g = Geometry(args["n0"], args["n1"], args["pixelsize"], args["detz"], K_UM,
+                 detoffset=args.get("detoffset", (0.0, 0.0)), detangles=args.get("detangles", (0.0, 0.0)))
+    true, bound = _block_true_coef(g), _block_bound_coef(g)
+    assert true <= bound, (true, bound)
+    assert bound <= 1.5 * slack * true, (true, bound)
