@@ -1046,10 +1046,26 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     } else if (it != h->tuned.end() && has_tile(tiles, it->second.tile)) {
         ch = it->second;
     } else if (h->autotune) {
+        // time the candidates on a prefix of the batch of at most ~2e11 pairs (2e10 for the
+        // FP64 kernels); a shot that crosses the cap is cut
+        const double npix = (double)h->g.n0 * (double)h->g.n1;
+        const double cap = split_path ? 2e11 : 2e10;
+        std::vector<int64_t> toff(1, 0);
+        for (int64_t s = 0; s < nshots; ++s) {
+            int64_t end = offsets[s + 1];
+            const double before = (double)offsets[s] * npix;
+            if (before + (double)(end - offsets[s]) * npix > cap) {
+                const int64_t room = std::max<int64_t>(kChunk, (int64_t)((cap - before) / npix));
+                toff.push_back(std::min(end, offsets[s] + room));
+                break;
+            }
+            toff.push_back(end);
+        }
+        const int64_t tshots = (int64_t)toff.size() - 1;
         Timer tm;
         double best = std::numeric_limits<double>::infinity();
         for (int v : tiles) {
-            rc = build_items(h, offsets, nshots, v, st);
+            rc = build_items(h, toff.data(), tshots, v, st);
             if (rc) return rc;
             const int occ = occupancy(h, c, v);
             std::vector<int> bpcs = {occ};
