@@ -13,7 +13,8 @@
 //               a = e^{i(phi_j - s)} (/ |r_p|)
 // in FP64 (d - |r_p| as (|r_j|^2 - 2 r_j . r_p)/(d + |r_p|), free of the cancellation of
 // two ~detector-distance terms that the reference's formula has); arrival times are float32
-// keys on FP32-split handles (the reference's "mixed" precision) and float64 on FP64 handles.
+// keys on FP32-split handles (the reference's "mixed" precision, which also takes the sine
+// and cosine of the range-reduced phase in FP32) and float64 on FP64 handles.
 //
 // Up to 20480 emitters (float32 keys; 12288 with float64 keys): k_time_fused, one workgroup
 // of 256 or 512 lanes per pixel, all in registers and LDS.
@@ -42,6 +43,11 @@ constexpr int kFusedMax64 = 512 * 24;
 
 namespace {
 // Arrival time t and amplitude (re, im) of emitter e at pixel r (|r| = R), FP64.
+// F32AMP (float32 arrival keys, the reference's mixed precision): the phase is
+// reduced to revolutions in FP64 and its sine and cosine taken in FP32
+// (v_sin_f32 / v_cos_f32, ~1e-7 absolute), as the reference's single path takes
+// __sincosf of a float phase.
+template <bool F32AMP>
 __device__ inline void arrival(const Geom& g, const double r[3], double R, const double* e, double T, double c,
                                double* t, double* re, double* im)
 {
@@ -64,10 +70,18 @@ __device__ inline void arrival(const Geom& g, const double r[3], double R, const
         ph = e[3] - s;
         if (g.scale) scale = rd;
     }
-    double sn, cs;
-    sincos(ph, &sn, &cs);
-    *re = cs * scale;
-    *im = sn * scale;
+    if (F32AMP) {
+        double rv = ph * kInv2Pi;
+        rv -= rint(rv);
+        const float fr = (float)rv;
+        *re = (double)__builtin_amdgcn_cosf(fr) * scale;
+        *im = (double)__builtin_amdgcn_sinf(fr) * scale;
+    } else {
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        *re = cs * scale;
+        *im = sn * scale;
+    }
 }
 
 // The arrival time alone (the same operations as arrival(); its unused sincos is dead code).
@@ -75,16 +89,17 @@ __device__ inline double arrival_time(const Geom& g, const double r[3], double R
                                       double c)
 {
     double t, re, im;
-    arrival(g, r, R, e, T, c, &t, &re, &im);
+    arrival<false>(g, r, R, e, T, c, &t, &re, &im);
     return t;
 }
 
 // The amplitude alone (the same operations as arrival()).
+template <bool F32AMP>
 __device__ inline void arrival_amp(const Geom& g, const double r[3], double R, const double* e, double* re,
                                    double* im)
 {
     double t;
-    arrival(g, r, R, e, 0.0, 1.0, &t, re, im);
+    arrival<F32AMP>(g, r, R, e, 0.0, 1.0, &t, re, im);
 }
 }  // namespace
 
@@ -102,7 +117,7 @@ __global__ __launch_bounds__(256) void k_time_prepare(Geom g, const double* __re
     pixel_pos(g, (double)(p / g.n1), (double)(p % g.n1), r);
     const double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     double t, re, im;
-    arrival(g, r, R, emit + 4 * j, times[j], c, &t, &re, &im);
+    arrival<sizeof(Tt) == 4>(g, r, R, emit + 4 * j, times[j], c, &t, &re, &im);
     const size_t o = (size_t)pl * n + j;
     keys[o] = (Tt)t;
     idx[o] = (int)j;
@@ -303,7 +318,7 @@ __global__ __launch_bounds__(BLOCK) void k_time_fused(Geom g, const double* __re
     for (int k = 0; k < cnt; ++k) {
         const int j = sidx[k];
         double re, im;
-        arrival_amp(g, r, R, emit + 4 * (int64_t)j, &re, &im);
+        arrival_amp<sizeof(Tt) == 4>(g, r, R, emit + 4 * (int64_t)j, &re, &im);
         const double tk = (double)skey[k];
         const double dk = exp(-(tk - tl) * inv_tau);
         pr = re + dk * pr;
