@@ -126,10 +126,16 @@ IDISIM_API const char *idisim_last_error(void);
    module compile) and the closure of cpu.py:6-40 (get_kernel). */
 IDISIM_API int idisim_create(idisim_handle **out, const idisim_geometry *geom, int precision, int device);
 
+/* Free the handle's device buffers and graph (no reference counterpart: cupy
+   frees the generator's buffers when they are garbage-collected). */
 IDISIM_API int idisim_destroy(idisim_handle *h);
 
+/* Set an IDISIM_OPT_* option; clears the handle's autotune cache (no
+   reference counterpart). */
 IDISIM_API int idisim_set_option(idisim_handle *h, int option, int64_t value);
 
+/* The plan of the handle's last forward / batched call (no reference
+   counterpart). */
 IDISIM_API int idisim_get_plan(const idisim_handle *h, idisim_plan *out);
 
 /* One shot.  Replaces the kernels wfkernelf / wfkerneld (sim.cu:141-149) as
