@@ -190,5 +190,17 @@ __host__ __device__ constexpr int tile_cols(int v) { return 16 * tile_py(v); }
 
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 constexpr int kChunk = 256;   // emitters staged in LDS per round
+constexpr int kSplitGrain = 64;  // finest emitter split over workgroups
+
+// Emitters per workgroup when n emitters are split over `splits` workgroups: whole
+// LDS chunks when every split gets at least one (a trailing partial chunk measured
+// slower), else multiples of 64 so that small n still spreads over many workgroups.
+__host__ __device__ inline int64_t split_size(int64_t n, int splits)
+{
+    const int64_t each = (n + splits - 1) / splits;
+    const int64_t grain = each >= kChunk ? kChunk : kSplitGrain;
+    const int64_t per = (each + grain - 1) / grain * grain;
+    return per > 0 ? per : grain;
+}
 
 }  // namespace idisim

@@ -404,7 +404,7 @@ int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
 
 std::vector<int> split_candidates(const idisim_handle* h, int64_t n, int tile)
 {
-    const int64_t maxs = std::max<int64_t>(1, (n + kChunk - 1) / kChunk);
+    const int64_t maxs = std::max<int64_t>(1, (n + kSplitGrain - 1) / kSplitGrain);
     const int tiles = ((h->g.n0 + tile_rows(tile) - 1) / tile_rows(tile)) *
                       ((h->g.n1 + tile_cols(tile) - 1) / tile_cols(tile));
     std::vector<int> out;
@@ -483,7 +483,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
             }
         ch.split = heuristic_split(h, c.n, ch.tile);
     }
-    const int64_t maxs = std::max<int64_t>(1, (c.n + kChunk - 1) / kChunk);
+    const int64_t maxs = std::max<int64_t>(1, (c.n + kSplitGrain - 1) / kSplitGrain);
     if (ch.split > maxs) ch.split = (int)maxs;
     int rc = run_forward(h, c, ch.tile, ch.split);
     if (rc) return rc;

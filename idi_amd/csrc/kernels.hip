@@ -900,10 +900,10 @@ static hipError_t launch_split_v(const Geom& g, const double* emit, const double
     constexpr int PX = tile_px(V), PY = tile_py(V);
     const int tiles_x = (g.n0 + 16 * PX - 1) / (16 * PX);
     const int tiles_y = (g.n1 + 16 * PY - 1) / (16 * PY);
-    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    const int64_t per = split_size(n, splits);
     dim3 grid(tiles_x * tiles_y, splits);
     hipLaunchKernelGGL((k_fwd_split<PX, PY, W, A, S>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
-                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+                       per, partial, out, kind);
     return hipGetLastError();
 }
 
@@ -995,10 +995,10 @@ static hipError_t launch_pair_q(const Geom& g, const double* emit, const double*
 {
     const int tiles_x = (g.n0 + 31) / 32;
     const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
-    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    const int64_t per = split_size(n, splits);
     dim3 grid(tiles_x * tiles_y, splits);
     hipLaunchKernelGGL((k_fwd_pair<QY, A, C>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
-                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+                       per, partial, out, kind);
     return hipGetLastError();
 }
 
@@ -1041,10 +1041,10 @@ static hipError_t launch_block_a(const Geom& g, const double* emit, const double
 {
     const int tiles_x = (g.n0 + 31) / 32;
     const int tiles_y = (g.n1 + 63) / 64;
-    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    const int64_t per = split_size(n, splits);
     dim3 grid(tiles_x * tiles_y, splits);
     hipLaunchKernelGGL((k_fwd_block<A>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
-                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+                       per, partial, out, kind);
     return hipGetLastError();
 }
 
@@ -1114,10 +1114,10 @@ static hipError_t launch_fp64_v(const Geom& g, const double* emit, const double*
     constexpr int PX = tile_px(V), PY = tile_py(V);
     const int tiles_x = (g.n0 + 16 * PX - 1) / (16 * PX);
     const int tiles_y = (g.n1 + 16 * PY - 1) / (16 * PY);
-    const int64_t per = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+    const int64_t per = split_size(n, splits);
     dim3 grid(tiles_x * tiles_y, splits);
     hipLaunchKernelGGL((k_fwd_fp64<PX, PY>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
-                       per > 0 ? per : (int64_t)kChunk, partial, out, kind);
+                       per, partial, out, kind);
     return hipGetLastError();
 }
 
