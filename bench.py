@@ -17,7 +17,8 @@ the CPU restatement on the full image), batched (256 shots, N_s ~ U{1000..50000}
 large (1e6 emitters in three balls, amplitudes U[0.5, 1], 1024^2 px of 50 um at
 0.5 m, 16 shots), adjoint (4096 emitters, 256^2 px of 75 um at 0.1 m), timedep (time-dependent
 simulation: 10000 emitters, 128^2 px, near field, float32 arrival keys), e2e (simulate() on the
-default workload's object per image: host or device draw, copies and kernel).
+default workload's object per image: host or device draw, copies and kernel), photons (randomphotons +
+chargesharing of a 512^2 image), correlate (streaming FFT correlator and q-corrected correlator).
 
 Prints ONE JSON line (rank 0).  --gpus must be 1: this build runs on one GPU.
 """
@@ -500,12 +501,150 @@ def bench_e2e(args):
             "device_draw": rates["device_draw"]}
 
 
+def _sized_cpu_sample(run, sizes, budget_s):
+    """Run `run(size)` on growing sizes until one takes about budget_s; returns (size, seconds, result)."""
+    res = None
+    for size in sizes:
+        t0 = time.perf_counter()
+        res = run(size)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4:
+            return size, dt, res
+    return size, dt, res
+
+
+def bench_photons(args):
+    """Photon sampling and detector charge sharing (drop-in for idi/simulation/common.py): a 512x512
+    speckle intensity image, randomphotons with 1e6 photons (Poisson on the device), chargesharing
+    with sigma 0.75 px (5x5 deposits per hit) and gain noise 0.1.  The unit is charge deposits
+    (hit x output pixel) per second."""
+    import torch
+
+    import oracle.photons as oph
+    from idi_amd.simulation import common
+
+    rng = np.random.default_rng(6000)
+    n0, nph, s, ns = 512, 1e6, 0.75, 0.1
+    w = int(np.ceil(2 * s))
+    inten = torch.from_numpy(rng.exponential(1.0, (n0, n0))).cuda()
+    seeds = iter(range(1, 1 << 30))
+
+    def step():
+        counts = common.randomphotons(inten, nph, rng=next(seeds))
+        return common.chargesharing(counts, s, ns, rng=next(seeds))
+
+    out = step()
+    torch.cuda.synchronize()
+    timer = StepTimer()
+    timer.run(step, args.warmup)
+    wall, ms = timer.run(step, args.steps)
+    deposits = nph * (2 * w + 1) ** 2
+    cpu = None
+    if not args.no_cpu:
+        counts = common.randomphotons(inten, nph, rng=12345).cpu().numpy()
+
+        def run(m):
+            sub = counts[:m, :m]
+            oph.chargesharing(sub, s, ns, 7)
+            return float(sub.sum())
+
+        m, dt, hits = _sized_cpu_sample(run, (32, 64, 96, 128, 192, 256), args.cpu_budget)
+        cpu = {"value": hits * (2 * w + 1) ** 2 / dt, "unit": "charge deposits/s", "cores": 1, "kind": "port",
+               "sample": f"{m}x{m} corner of one photon image ({hits:.0f} hits, {dt:.1f} s); the per-hit restatement "
+                         f"of common.py:26-67 in Python with math.erf (oracle/photons.py)"}
+    return {"metric": "charge deposits/s [photon sampling + charge sharing]", "value": deposits * args.steps / wall,
+            "unit": "charge deposits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+            "config": {"workload": "512x512 exponential speckle intensity, randomphotons 1e6 photons (torch.poisson on "
+                                   "the device), chargesharing sigma 0.75 px, gain noise 0.1 (idisim_chargesharing: "
+                                   "one thread per output pixel gathers the hits of its 5x5 neighbourhood)",
+                       "photons": nph, "deposits_per_step": deposits, "event_ms": float(np.mean(ms))},
+            "roofline": None, "cpu_baseline": cpu}
+
+
+def bench_correlate(args):
+    """Correlators of photon images (drop-in for idi/reconstruction/cusimple.py and cucor.py): the
+    streaming FFT correlator (correlator2d.add: normalisation, rocFFT autocorrelation of the image,
+    masked divide and Welford update) on 512x512 float32 images, and the q-corrected pair correlator
+    (corrfunction) on 256x256 images at z = 2000 px with maxq 64."""
+    import torch
+
+    import oracle.correlate as ocr
+    from idi_amd.reconstruction import qcor, simple
+
+    rng = np.random.default_rng(7000)
+    n = 512
+    imgs = [torch.from_numpy(rng.poisson(2.0, (n, n)).astype(np.float32)).cuda() for _ in range(4)]
+    c2d = simple.correlator2d()
+    it = iter(range(1 << 30))
+
+    def step():
+        c2d.add(imgs[next(it) % len(imgs)])
+
+    step()
+    torch.cuda.synchronize()
+    timer = StepTimer()
+    timer.run(step, args.warmup)
+    wall, ms = timer.run(step, args.steps)
+
+    nq, zq, maxq = 256, 2000.0, 64
+    qimgs = [torch.from_numpy(rng.poisson(2.0, (nq, nq)).astype(np.float32)).cuda() for _ in range(2)]
+    with qcor.corrfunction((nq, nq), zq, maxq) as qc:
+        qc(qimgs[0])
+        zd, xc, yc, qmax, maxdqz = qcor.geometry((nq, nq), zq, maxq)
+        dzd = torch.from_numpy(zd).cuda()
+        qout = torch.empty((2 * maxdqz, 2 * qmax, 2 * qmax), dtype=torch.float64, device="cuda")
+        from idi_amd import _lib
+
+        qtimer = StepTimer()
+        qtimer.run(lambda: _lib.qcorr(dzd, qimgs[0], xc, yc, zq, qmax, maxdqz, qout), 2)
+        qwall, qms = qtimer.run(lambda: _lib.qcorr(dzd, qimgs[0], xc, yc, zq, qmax, maxdqz, qout), 10)
+    qpairs = sum(min(nq - r, qmax) for r in range(nq)) * sum(nq - abs(dc) for dc in range(-(qmax - 1), qmax))
+    cpu = qcpu = None
+    if not args.no_cpu:
+        import scipy.signal
+
+        host = imgs[0].cpu().numpy()
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < args.cpu_budget / 2:
+            scipy.signal.correlate(host, host, method="fft")
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": reps / dt, "unit": "images/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} autocorrelations of one {n}x{n} image with scipy.signal.correlate(method='fft') "
+                         f"({dt:.1f} s), the comparator of the reference's tests/test_recon.py:106-123"}
+        qhost = qimgs[0].cpu().numpy()
+
+        def run(rows):
+            ocr.qcorr(zd[:rows], qhost[:rows], xc, yc, zq, qmax, maxdqz)
+            return rows
+
+        rows, qdt, _ = _sized_cpu_sample(run, (4, 8, 16, 32, 64, 128, 256), args.cpu_budget / 2)
+        sub_pairs = sum(min(rows - r, qmax) for r in range(rows)) * sum(nq - abs(dc) for dc in range(-(qmax - 1), qmax))
+        qcpu = {"value": sub_pairs / qdt, "unit": "pixel pairs/s", "cores": 1, "kind": "port",
+                "sample": f"first {rows} of {nq} rows ({sub_pairs:.3e} pairs, {qdt:.1f} s); NumPy restatement of "
+                          f"cucor.py:54-96 (oracle/correlate.py)"}
+    return {"metric": "images/s [streaming FFT correlator] + pixel pairs/s [q-corrected correlator]",
+            "value": args.steps / wall, "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"correlator2d.add on {n}x{n} Poisson(2) float32 images (FFT length "
+                                   f"{simple._fastlen(2 * n)}); corrfunction on {nq}x{nq} images, z {zq:.0f} px, maxq {maxq}",
+                       "event_ms": float(np.mean(ms))},
+            "qcorr": {"value": qpairs * 10 / qwall, "unit": "pixel pairs/s", "ms_per_image": qwall * 1e3 / 10,
+                      "pairs_per_image": qpairs, "cpu_baseline": qcpu},
+            "roofline": None, "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep", "e2e"])
+    ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep", "e2e", "photons",
+                                                         "correlate"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
@@ -523,7 +662,8 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     res = {"single": bench_single, "batched": bench_batched, "large": bench_large,
-           "adjoint": bench_adjoint, "timedep": bench_timedep, "c1": bench_c1, "e2e": bench_e2e}[args.workload](args)
+           "adjoint": bench_adjoint, "timedep": bench_timedep, "c1": bench_c1, "e2e": bench_e2e,
+           "photons": bench_photons, "correlate": bench_correlate}[args.workload](args)
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.rand(n, device="cuda")
