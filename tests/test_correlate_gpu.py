@@ -87,6 +87,26 @@ def test_qcorr_matches_restatement():
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
 
 
+def test_qcorr_workgroup_histogram_geometry():
+    """A 64 x 64 image at z = 500 px with maxq 32 (the workgroup-private LDS histogram with its dqx
+    window, and the global atomics for pairs outside it) against the restatement; the same image
+    off-centre (beam centre at a corner), where the windows are widest."""
+    import torch
+
+    from idi_amd import _lib
+    from idi_amd.reconstruction.qcor import geometry
+    from oracle.correlate import qcorr
+
+    rng = np.random.default_rng(9)
+    img = rng.poisson(2.0, (64, 64)).astype(np.float32)
+    for xcen, ycen in ((None, None), (0.0, 0.0)):
+        zd, xc, yc, qmax, maxdqz = geometry(img.shape, 500.0, 32, xcen, ycen)
+        out = torch.empty((2 * maxdqz, 2 * qmax, 2 * qmax), dtype=torch.float64, device="cuda")
+        _lib.qcorr(torch.from_numpy(zd).cuda(), torch.from_numpy(img).cuda(), xc, yc, 500.0, qmax, maxdqz, out)
+        want = qcorr(zd, img, xc, yc, 500.0, qmax, maxdqz)
+        np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-12, atol=1e-12)
+
+
 def test_cucor_reference_case():
     """tests/test_recon.py:84-93 of the reference."""
     from idi_amd.reconstruction import cucor
