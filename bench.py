@@ -18,7 +18,8 @@ large (1e6 emitters in three balls, amplitudes U[0.5, 1], 1024^2 px of 50 um at
 0.5 m, 16 shots), adjoint (4096 emitters, 256^2 px of 75 um at 0.1 m), timedep (time-dependent
 simulation: 10000 emitters, 128^2 px, near field, float32 arrival keys), e2e (simulate() on the
 default workload's object per image: host or device draw, copies and kernel), photons (randomphotons +
-chargesharing of a 512^2 image), correlate (streaming FFT correlator and q-corrected correlator).
+chargesharing of a 512^2 image), correlate (streaming FFT correlator and q-corrected correlator), generate (emitters drawn on
+the device).
 
 Prints ONE JSON line (rank 0).  --gpus must be 1: this build runs on one GPU.
 """
@@ -638,13 +639,71 @@ def bench_correlate(args):
             "roofline": None, "cpu_baseline": cpu}
 
 
+def bench_generate(args):
+    """Emitters drawn on the device (drop-in for the draws of idi/simulation/simobj.py): 6.5e6 emitters
+    uniform in a 30 nm ball with phases (the emitter total of the batched 256-shot workload), 1e6 in a
+    Gaussian cloud, and 100000 of the 155331 simple-cubic sites of a 10 nm ball without replacement.
+    The unit is emitters (x, y, z, phase) written per second; the CPU baseline is the numpy draw of the
+    same simobj objects on the host."""
+    import torch
+
+    from idi_amd import _lib
+    from idi_amd.simulation import simobj
+
+    nb, ng = 6_500_000, 1_000_000
+    dball = torch.empty((nb, 4), dtype=torch.float64, device="cuda")
+    dgauss = torch.empty((ng, 4), dtype=torch.float64, device="cuda")
+    m = int(np.ceil(0.01 / 3e-4))
+    g = np.arange(-m, m + 1) * 3e-4
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pts = pts[np.linalg.norm(pts, axis=1) <= 0.01 * (1 + 1e-12)]
+    sampler = _lib.ChoiceSampler(pts)
+    dchoice = torch.empty((100_000, 4), dtype=torch.float64, device="cuda")
+    seeds = iter(range(1, 1 << 30))
+    sigma = np.full(3, 0.02 / (2 * np.sqrt(2 * np.log(2))))
+    rates = {}
+    for name, n, fn in (("ball", nb, lambda: _lib.generate_ball(dball, next(seeds), 0, 0.03)),
+                        ("gauss", ng, lambda: _lib.generate_gauss(dgauss, next(seeds), sigma)),
+                        ("choice", 100_000, lambda: sampler.draw(dchoice, next(seeds)))):
+        fn()
+        torch.cuda.synchronize()
+        timer = StepTimer()
+        timer.run(fn, args.warmup)
+        wall, ms = timer.run(fn, args.steps)
+        rates[name] = {"value": n * args.steps / wall, "ms_per_draw": wall * 1e3 / args.steps}
+    cpu = None
+    if not args.no_cpu:
+        ball = simobj.sphere(8048.0, nb, 0.03)
+        ball.rng = np.random.default_rng(1)
+        t0 = time.perf_counter()
+        ball.get()
+        dt = time.perf_counter() - t0
+        cpu = {"value": nb / dt, "unit": "emitters/s", "cores": 1, "kind": "port",
+               "sample": f"one simobj.sphere(N=6.5e6, r=30 nm).get() with numpy on the host ({dt:.1f} s)"}
+        crys = simobj.sc(8048.0, 100_000, 3e-4, repeats=[2 * m + 1] * 3)
+        crys._allpos = pts
+        crys.rng = np.random.default_rng(2)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            crys.get()
+        rates["choice"]["cpu"] = 5 * 100_000 / (time.perf_counter() - t0)
+    return {"metric": "emitters/s [on-device emitter generation]", "value": rates["ball"]["value"],
+            "unit": "emitters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": rates["ball"]["ms_per_draw"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+            "config": {"workload": "6.5e6 emitters uniform in a 30 nm ball with phases (Philox4x32-10, "
+                                   "idisim_generate_ball); also a 1e6 Gaussian cloud and the 100000-of-155331 "
+                                   "site choice", "emitters": nb},
+            "gauss": rates["gauss"], "choice": rates["choice"], "roofline": None, "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep", "e2e", "photons",
-                                                         "correlate"])
+                                                         "correlate", "generate"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
@@ -663,7 +722,7 @@ def main():
         sys.exit(1)
     res = {"single": bench_single, "batched": bench_batched, "large": bench_large,
            "adjoint": bench_adjoint, "timedep": bench_timedep, "c1": bench_c1, "e2e": bench_e2e,
-           "photons": bench_photons, "correlate": bench_correlate}[args.workload](args)
+           "photons": bench_photons, "correlate": bench_correlate, "generate": bench_generate}[args.workload](args)
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.rand(n, device="cuda")
