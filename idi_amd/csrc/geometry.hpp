@@ -168,17 +168,31 @@ struct WorkItem {
 // second centre is reached by a first-order rotation, so no remainder is dropped.
 // 8: four mirror pairs about a 2 x 4 block per lane (32x64 pixels), a centre per
 // pair, one sincos of a centre per block.
-constexpr int kNumTiles = 9;
+// 9..12: the mirror pairs of 5 (9, 10) and the centred pairs of 7 (11, 12) with the
+// per-pair sincos of delta_k taken from a reference: every wave owns a 16 x 16
+// pixel block, the phasor of delta_k at the block centre is formed once per emitter
+// and wave, and each lane rotates it to its own pair to first (9, 11) or second
+// (10, 12) order.
+constexpr int kNumTiles = 13;
 constexpr int kTileMfma = 4;
 constexpr int kTilePair1 = 5;
 constexpr int kTilePair2 = 6;
 constexpr int kTilePairC = 7;
 constexpr int kTileBlock = 8;
-__host__ __device__ constexpr bool tile_pair(int v)
+constexpr int kTileRef1 = 9;
+constexpr int kTileRef2 = 10;
+constexpr int kTileRefC1 = 11;
+constexpr int kTileRefC2 = 12;
+__host__ __device__ constexpr bool tile_pair(int v) { return v >= kTilePair1 && v <= kTileRefC2; }
+__host__ __device__ constexpr bool tile_pair_centred(int v)
 {
-    return v == kTilePair1 || v == kTilePair2 || v == kTilePairC || v == kTileBlock;
+    return v == kTilePairC || v == kTileRefC1 || v == kTileRefC2;
 }
-__host__ __device__ constexpr bool tile_pair_centred(int v) { return v == kTilePairC; }
+// order of the rotation from the wave reference to the lane's delta_k (0: own sincos)
+__host__ __device__ constexpr int tile_ref_order(int v)
+{
+    return (v == kTileRef1 || v == kTileRefC1) ? 1 : ((v == kTileRef2 || v == kTileRefC2) ? 2 : 0);
+}
 __host__ __device__ constexpr int pair_quads(int v) { return v == kTilePair2 ? 2 : 1; }
 __host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileMfma ? 2 : 1; }
 __host__ __device__ constexpr int tile_py(int v)

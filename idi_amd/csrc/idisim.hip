@@ -89,6 +89,7 @@ struct idisim_handle {
     double Rmin = 0.0, Rmax = 0.0;
     double pair_coef = 0.0;   // mirror-pair remainder per unit emitter extent (rev per length)
     double block_coef = 0.0;  // bound of |m_k - m_1| of the 2 x 4 block pairs per unit extent
+    double ref_coef = 0.0;    // bound of |d_k - D_k| (lane pair vs wave reference) per unit extent
     double rho = 0.0;   // emitter extent of the current call
     // device scratch
     void* scratch = nullptr;
@@ -187,6 +188,25 @@ void planner_tables(idisim_handle* h)
     const double sb = std::min(1.0, ebl / h->Rmin);
     h->block_coef = 1.05 * 1.5 * (1.0 + (2.0 + std::sqrt(2.0)) * sb + 6.0 * sb * sb) * g.krev * g.pix * g.pix /
                     (h->Rmin * h->Rmin);
+    // Wave references (variants 9..12): d_k(p) - D_k = int_0^1 int_-1^1 D2f[a_k, o] dt ds along the
+    // segment from the block centre P to the quad centre p, o = p - P, |a_k| = sqrt(2) / 2 pixel.
+    // D2f is a symmetric bilinear form with |D2f[v, v]| <= krev pix^2 |v|^2 C(s) / R^2,
+    // C(s) = 1 + (2 + sqrt 2) s + 6 s^2 (as above), so |D2f[a, o]| <= krev pix^2 |a| |o| C(s) / R^2.
+    // In a 16 x 16 pixel wave block |o| <= 7 sqrt 2 pixels.  The segments and the block centres
+    // reach to the edge of the last (padded) 32 x 32 tile, so s and R are taken over the padded
+    // detector, R at its point nearest the foot of the detector normal.
+    {
+        const double x0 = (-1.0 - g.half0) * g.pix + g.off0;
+        const double x1 = (32.0 * ((g.n0 + 31) / 32) - g.half0) * g.pix + g.off0;
+        const double y0 = (-1.0 - g.half1) * g.pix + g.off1;
+        const double y1 = (32.0 * ((g.n1 + 31) / 32) - g.half1) * g.pix + g.off1;
+        const double ex = std::max(std::fabs(x0), std::fabs(x1)), ey = std::max(std::fabs(y0), std::fabs(y1));
+        const double nx = std::min(std::max(0.0, x0), x1), ny = std::min(std::max(0.0, y0), y1);
+        const double Rp = std::min(h->Rmin, std::sqrt(nx * nx + ny * ny + g.detz * g.detz));
+        const double sr = std::min(1.0, std::max(ex, ey) / Rp);
+        h->ref_coef = 1.05 * (1.0 + (2.0 + std::sqrt(2.0)) * sr + 6.0 * sr * sr) * g.krev * g.pix * g.pix *
+                      (0.5 * std::sqrt(2.0)) * (7.0 * std::sqrt(2.0)) / (Rp * Rp);
+    }
 }
 
 // Error (revolutions) of folding W_cj into A_cj, i.e. of replacing lam_p W_cj
@@ -259,8 +279,16 @@ double plan_bound(const idisim_handle* h, int v, double rho)
         } else {
             e += rem;  // dropped remainder e_k
         }
-        e += kSinCosErrRev;  // second sincos (delta_k)
+        e += kSinCosErrRev;  // second sincos (delta_k, or the reference E_k)
         e += 2.0 * u24;      // e^{i tau_c} (cos, sin) delta_k products, as phase
+        if (const int ord = tile_ref_order(v)) {
+            // rotation from the wave reference, t = 2 pi (d_k - D_k) . r': truncation t^2 / 2
+            // (first order) or |t|^3 / 6 (second order), t's own FP32 rounding, and the
+            // rounding of D_k and of the rotated (cos, sin)
+            const double t = 2.0 * M_PI * h->ref_coef * rho * (1.0 + sig);
+            const double trunc = ord == 1 ? 0.5 * t * t : t * t * t / 6.0;
+            e += (trunc + 4.0 * u24 * t) * kInv2Pi + 2.0 * u24 * B + 2.0 * u24;
+        }
     }
     return e;
 }
@@ -476,7 +504,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     } else {
         // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -1091,7 +1119,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;

@@ -218,19 +218,68 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
 // e^{i tau_c} e^{-i Delta} with Delta = 2 pi (m_0 - m_1) . r' (a few 1e-4 rad),
 // taken to first order as (1 - i Delta): 5 more FMA per quad, an error of
 // Delta^2 / 2 that the host planner bounds.
-template <int QY, bool HAS_AMP, bool CENTRED>
+// REF (1 or 2): the sincos of delta_k comes from a reference.  Every wave owns a
+// 16 x 16QY pixel block (8 x 8 lanes) and D_k, the half difference d_k evaluated at
+// the block centre (pixels at +/- (1/2, 1/2) about it); the record stage forms
+// E_k = e^{i 2 pi D_k . r'} once per emitter and wave (8 sincos per emitter, 1/128
+// per pixel pair), and each lane reaches its own pair by the rotation
+// e^{i t}, t = 2 pi (d_k - D_k) . r' (|t| a few 1e-3 rad), taken to first order
+// (1 + i t; REF 1) or second order (1 + i t - t^2 / 2; REF 2).  Per quad and
+// emitter that is 2 transcendentals and 21 (REF 1) or 27 (REF 2) other VALU where
+// REF 0 spends 6 and 17.  The host planner bounds t^2 / 2 (REF 1) or |t|^3 / 6
+// (REF 2).  sD holds the four waves' D_k (FP32, 24 floats) and sE their phasors,
+// (cos, sin) of both pairs per float4, with REF 2 a second float4 scaled by sqrt 2.
+template <int REF>
+__device__ __forceinline__ void pair_lane(int tid, int& lx, int& ly)
+{
+    if (REF == 0) {
+        lx = tid >> 4;
+        ly = tid & 15;
+    } else {
+        const int w = tid >> 6, l = tid & 63;
+        lx = ((w >> 1) << 3) | (l >> 3);
+        ly = ((w & 1) << 3) | (l & 7);
+    }
+}
+
+template <int QY, bool HAS_AMP, bool CENTRED, int REF>
 __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restrict__ emit,
                                           const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0,
-                                          int ty0, float4* s0, float2* s1, float (&acc)[QY][2][4],
-                                          float (&cmp)[QY][2][4])
+                                          int ty0, float4* s0, float2* s1, float* sD, float4* sE,
+                                          float (&acc)[QY][2][4], float (&cmp)[QY][2][4])
 {
     constexpr int TX = 32, TY = 32 * QY;
     const int tid = threadIdx.x;
-    const int lx = tid >> 4, ly = tid & 15;
+    int lx, ly;
+    pair_lane<REF>(tid, lx, ly);
 
     double ccx, ccy;
     tile_centre(g, tx0, ty0, TX, TY, ccx, ccy);
     const Frame c = pixel_frame(g, ccx, ccy);
+
+    // the wave's reference half differences D_k (as the FP32 values the records use)
+    double Dref[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    if (REF != 0) {
+        const int w = tid >> 6;
+        const double px = (double)(tx0 + 16 * (w >> 1)) + 7.5;
+        const double py = (double)(ty0 + 16 * QY * (w & 1)) + (8.0 * QY - 0.5);
+        const Frame f00 = pixel_frame(g, px - 0.5, py - 0.5), f11 = pixel_frame(g, px + 0.5, py + 0.5);
+        const Frame f10 = pixel_frame(g, px + 0.5, py - 0.5), f01 = pixel_frame(g, px - 0.5, py + 0.5);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double q00 = -g.krev * (f00.u[a] - c.u[a]), q11 = -g.krev * (f11.u[a] - c.u[a]);
+            const double q10 = -g.krev * (f10.u[a] - c.u[a]), q01 = -g.krev * (f01.u[a] - c.u[a]);
+            const float D0 = (float)(0.5 * (q11 - q00)), D1 = (float)(0.5 * (q01 - q10));
+            Dref[0][a] = (double)D0;
+            Dref[1][a] = (double)D1;
+            if ((tid & 63) == 0) {
+                sD[(2 * w) * 3 + a] = D0;
+                sD[(2 * w + 1) * 3 + a] = D1;
+            }
+        }
+    }
+    // REF 1: t in radians; REF 2: t / sqrt 2, so that its square is t^2 / 2
+    const double dscale = REF == 1 ? kTwoPi : kTwoPi * 0.70710678118654752440;
 
     float mc[QY][3], d[QY][2][3], dv[QY][3];
 #pragma unroll
@@ -251,8 +300,10 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             const double m1 = 0.5 * (dq[1][0][a] + dq[0][1][a]);
             mc[q][a] = (float)(CENTRED ? m0 : 0.5 * (m0 + m1));
             dv[q][a] = (float)(kTwoPi * (m0 - m1));
-            d[q][0][a] = (float)(0.5 * (dq[1][1][a] - dq[0][0][a]));
-            d[q][1][a] = (float)(0.5 * (dq[0][1][a] - dq[1][0][a]));
+            const double h0 = 0.5 * (dq[1][1][a] - dq[0][0][a]);
+            const double h1 = 0.5 * (dq[0][1][a] - dq[1][0][a]);
+            d[q][0][a] = (float)(REF ? dscale * (h0 - Dref[0][a]) : h0);
+            d[q][1][a] = (float)(REF ? dscale * (h1 - Dref[1][a]) : h1);
         }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -263,6 +314,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             }
         }
     }
+    const float4* eref = sE + (REF * (tid >> 6)) * kChunk;  // this wave's phasors
 
     for (int64_t base = e0; base < e1; base += kChunk) {
         const int cnt = (e1 - base) < (int64_t)kChunk ? (int)(e1 - base) : kChunk;
@@ -274,6 +326,27 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], ea);
             s0[tid] = make_float4(r.x, r.y, r.z, r.A);
             s1[tid] = make_float2(r.W, r.amp);
+            if (REF != 0) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    float ec[2], es[2];
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const float* D = sD + (2 * w + k) * 3;
+                        float dl = D[0] * r.x;
+                        dl = fmaf(D[1], r.y, dl);
+                        dl = fmaf(D[2], r.z, dl);
+                        es[k] = __builtin_amdgcn_sinf(dl);
+                        ec[k] = __builtin_amdgcn_cosf(dl);
+                    }
+                    sE[(REF * w) * kChunk + tid] = make_float4(ec[0], es[0], ec[1], es[1]);
+                    if (REF == 2) {
+                        constexpr float r2 = 1.41421356237309504880f;
+                        sE[(REF * w + 1) * kChunk + tid] =
+                            make_float4(r2 * ec[0], r2 * es[0], r2 * ec[1], r2 * es[1]);
+                    }
+                }
+            }
         }
         __syncthreads();
 
@@ -288,6 +361,9 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
         for (int e = 0; e < cnt; ++e) {
             const float4 r0 = s0[e];
             const float am = HAS_AMP ? s1[e].y : 1.f;
+            float4 E = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E;
+            if (REF != 0) E = eref[e];
+            if (REF == 2) E2 = eref[kChunk + e];
 #pragma unroll
             for (int q = 0; q < QY; ++q) {
                 float t = fmaf(mc[q][0], r0.x, r0.w);
@@ -312,8 +388,22 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                     float dl = d[q][k][0] * r0.x;
                     dl = fmaf(d[q][k][1], r0.y, dl);
                     dl = fmaf(d[q][k][2], r0.z, dl);
-                    const float sd = __builtin_amdgcn_sinf(dl);
-                    const float cd = __builtin_amdgcn_cosf(dl);
+                    float sd, cd;
+                    if (REF == 0) {
+                        sd = __builtin_amdgcn_sinf(dl);
+                        cd = __builtin_amdgcn_cosf(dl);
+                    } else {
+                        const float ec = k ? E.z : E.x, es = k ? E.w : E.y;
+                        if (REF == 1) {  // (ec + i es)(1 + i t), t = dl
+                            cd = fmaf(-dl, es, ec);
+                            sd = fmaf(dl, ec, es);
+                        } else {         // (ec + i es)(1 - u^2 + i sqrt2 u), u = dl
+                            const float sec = k ? E2.z : E2.x, ses = k ? E2.w : E2.y;
+                            const float uu = dl * dl;
+                            cd = fmaf(-dl, ses, fmaf(-uu, ec, ec));
+                            sd = fmaf(dl, sec, fmaf(-uu, es, es));
+                        }
+                    }
                     su[q][k][0] = fmaf(pc[k], cd, su[q][k][0]);
                     su[q][k][1] = fmaf(ps[k], cd, su[q][k][1]);
                     su[q][k][2] = fmaf(pc[k], sd, su[q][k][2]);
@@ -486,7 +576,7 @@ __device__ __forceinline__ void split_hi_lo(double v, float& hi, float& lo)
     lo = (float)((double)hi - v);
 }
 
-template <int QY, bool HAS_AMP, bool CENTRED>
+template <int QY, bool HAS_AMP, bool CENTRED, int REF>
 __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __restrict__ emit,
                                                      const double* __restrict__ amp, int64_t n, int tiles_y,
                                                      int64_t per_split, float4* __restrict__ partial, void* out,
@@ -495,15 +585,18 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
     constexpr int TX = 32, TY = 32 * QY;
     __shared__ float4 s0[kChunk];
     __shared__ float2 s1[kChunk];
+    __shared__ float sD[REF ? 24 : 1];
+    __shared__ float4 sE[REF ? 4 * REF * kChunk : 1];
     const int tile = blockIdx.x;
     const int split = blockIdx.y;
     const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
     const int64_t e0 = (int64_t)split * per_split;
     const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
     float acc[QY][2][4], cmp[QY][2][4];
-    pair_tile<QY, HAS_AMP, CENTRED>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, acc, cmp);
+    pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, sD, sE, acc, cmp);
 
-    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    int lx, ly;
+    pair_lane<REF>(threadIdx.x, lx, ly);
     const size_t npix = (size_t)g.n0 * g.n1;
 #pragma unroll
     for (int q = 0; q < QY; ++q) {
@@ -611,7 +704,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_block(Geom g, const double* 
 }
 
 // Persistent batched form of k_fwd_pair (the queue protocol of k_batched_split).
-template <int QY, bool HAS_AMP, bool CENTRED>
+template <int QY, bool HAS_AMP, bool CENTRED, int REF>
 __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* __restrict__ emit,
                                                          const double* __restrict__ amp,
                                                          const WorkItem* __restrict__ items, int nitems,
@@ -620,8 +713,11 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* _
     constexpr int TX = 32, TY = 32 * QY;
     __shared__ float4 s0[kChunk];
     __shared__ float2 s1[kChunk];
+    __shared__ float sD[REF ? 24 : 1];
+    __shared__ float4 sE[REF ? 4 * REF * kChunk : 1];
     __shared__ int s_item;
-    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    int lx, ly;
+    pair_lane<REF>(threadIdx.x, lx, ly);
     const size_t npix = (size_t)g.n0 * g.n1;
     for (;;) {
         __syncthreads();
@@ -632,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* _
         const WorkItem w = items[it];
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         float acc[QY][2][4], cmp[QY][2][4];
-        pair_tile<QY, HAS_AMP, CENTRED>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, acc, cmp);
+        pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, sD, sE, acc, cmp);
         char* outs = static_cast<char*>(out) +
                      (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
 #pragma unroll
@@ -989,7 +1085,7 @@ int occupancy_batched_split(int v, bool w, bool a, int s)
     return 1;
 }
 
-template <int QY, bool A, bool C>
+template <int QY, bool A, bool C, int R>
 static hipError_t launch_pair_q(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
                                 float4* partial, void* out, int kind, hipStream_t st)
 {
@@ -997,42 +1093,51 @@ static hipError_t launch_pair_q(const Geom& g, const double* emit, const double*
     const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
     const int64_t per = split_size(n, splits);
     dim3 grid(tiles_x * tiles_y, splits);
-    hipLaunchKernelGGL((k_fwd_pair<QY, A, C>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
+    hipLaunchKernelGGL((k_fwd_pair<QY, A, C, R>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
                        per, partial, out, kind);
     return hipGetLastError();
 }
 
-template <int QY, bool A, bool C>
+template <int QY, bool A, bool C, int R>
 static hipError_t launch_batched_pair_q(const Geom& g, const double* emit, const double* amp,
                                         const WorkItem* items, int nitems, int* counter, int nblocks, void* out,
                                         int kind, hipStream_t st)
 {
     const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL((k_batched_pair<QY, A, C>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp, items, nitems,
+    hipLaunchKernelGGL((k_batched_pair<QY, A, C, R>), dim3(nblocks), dim3(kBlock), 0, st, g, emit, amp, items, nitems,
                        counter, tiles_y, out, kind);
     return hipGetLastError();
 }
 
-template <int QY, bool A, bool C>
+template <int QY, bool A, bool C, int R>
 static int occupancy_pair_q()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_pair<QY, A, C>, kBlock, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched_pair<QY, A, C, R>, kBlock, 0) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }
 
-// (quads per lane, amplitudes, centred) -> instantiation; variant 7 has one quad per lane
-#define IDISIM_DISPATCH_PAIR(FN, ...)                                      \
-    switch ((pair_quads(v) << 2) | ((int)tile_pair_centred(v) << 1) | (int)a) { \
-        case 4: return FN<1, false, false>(__VA_ARGS__);                  \
-        case 5: return FN<1, true, false>(__VA_ARGS__);                   \
-        case 6: return FN<1, false, true>(__VA_ARGS__);                   \
-        case 7: return FN<1, true, true>(__VA_ARGS__);                    \
-        case 8: return FN<2, false, false>(__VA_ARGS__);                  \
-        case 9: return FN<2, true, false>(__VA_ARGS__);                   \
-        default: break;                                                   \
+// (quads per lane, reference order, centred, amplitudes) -> instantiation; only
+// variant 6 has two quads per lane
+#define IDISIM_DISPATCH_PAIR(FN, ...)                                                          \
+    switch ((pair_quads(v) << 4) | (tile_ref_order(v) << 2) | ((int)tile_pair_centred(v) << 1) | (int)a) { \
+        case 16: return FN<1, false, false, 0>(__VA_ARGS__);                                  \
+        case 17: return FN<1, true, false, 0>(__VA_ARGS__);                                   \
+        case 18: return FN<1, false, true, 0>(__VA_ARGS__);                                   \
+        case 19: return FN<1, true, true, 0>(__VA_ARGS__);                                    \
+        case 20: return FN<1, false, false, 1>(__VA_ARGS__);                                  \
+        case 21: return FN<1, true, false, 1>(__VA_ARGS__);                                   \
+        case 22: return FN<1, false, true, 1>(__VA_ARGS__);                                   \
+        case 23: return FN<1, true, true, 1>(__VA_ARGS__);                                    \
+        case 24: return FN<1, false, false, 2>(__VA_ARGS__);                                  \
+        case 25: return FN<1, true, false, 2>(__VA_ARGS__);                                   \
+        case 26: return FN<1, false, true, 2>(__VA_ARGS__);                                   \
+        case 27: return FN<1, true, true, 2>(__VA_ARGS__);                                    \
+        case 32: return FN<2, false, false, 0>(__VA_ARGS__);                                  \
+        case 33: return FN<2, true, false, 0>(__VA_ARGS__);                                   \
+        default: break;                                                                       \
     }
 
 template <bool A>

@@ -178,7 +178,7 @@ def test_empty_and_errors(lib):
     with pytest.raises(ValueError):
         h.forward(torch.zeros((4, 3), dtype=torch.float64, device="cuda"))
     with pytest.raises(lib.IdisimError):
-        h.set_option(lib.OPT_TILE, 9)
+        h.set_option(lib.OPT_TILE, 13)
     with pytest.raises(lib.IdisimError):
         h.set_option(999, 1)
     with pytest.raises(lib.IdisimError):
@@ -247,7 +247,8 @@ def test_autotune_cache_across_extents(lib, oracle_mod):
 @pytest.mark.parametrize("mode", ["nf", "ff", "ff_secondorder", "nf_scale_amp", "nf_rot", "ff_rot"])
 def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
     """Mirror-pair tiles (5: one 2 x 2 quad per lane, 6: two, 7: one with a centre per pair, 8: four
-    pairs about a 2 x 4 block) on the pixel pitch and distance of
+    pairs about a 2 x 4 block, 9..12: 5 and 7 with the pair phasors rotated from a per-wave reference
+    to first or second order) on the pixel pitch and distance of
     the 100k-site image (75 um at 0.2 m, sites of a 10 nm ball), on an odd detector so that quads
     reach past the last row and column; every emitter split and output kind; a rotated and
     offset detector."""
@@ -261,7 +262,7 @@ def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
     at, _ = sc_ball(rng, 3000, 3e-4, 0.01)
     amp = rng.uniform(0.5, 1.0, len(at)) if "amp" in mode else None
     ref = oracle_mod.field_c(g, at, amp)
-    for tile in (5, 6, 7, 8):
+    for tile in (5, 6, 7, 8, 9, 10, 11, 12):
         for split in (1, 3):
             f, h = run(lib, g, at, amp, tile=tile, split=split)
             p = h.plan()
@@ -275,9 +276,10 @@ def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
 
 def test_pair_tiles_eligibility(lib, oracle_mod):
     """Pixels of 100 um at 0.1 m and a 30 nm ball: the pair remainder (~1.2e-5 rev) exceeds the
-    5e-6 rev tolerance, so the planner never picks tile 5 or 6 and a forced one reports its bound;
-    tile 7 (a centre per pair) drops no remainder and qualifies; the per-term Kahan mode has no
-    pair kernel."""
+    5e-6 rev tolerance, so the planner never picks tiles 5, 6, 9 or 10 and a forced one reports its
+    bound; tile 7 (a centre per pair) drops no remainder and qualifies; so does tile 12, whose
+    second-order rotation from the wave reference stays ~1e-7 rev, while the first-order tile 11
+    (~1.5e-5 rev) does not; the per-term Kahan mode has no pair kernel."""
     from oracle import Geometry
 
     rng = np.random.default_rng(46)
@@ -286,14 +288,15 @@ def test_pair_tiles_eligibility(lib, oracle_mod):
     at = atoms_ball(rng, 600, 0.03)
     ref = oracle_mod.field_c(g, at)
     _, h = run(lib, g, at)
-    assert h.plan()["tile"] not in (5, 6)
-    f, h = run(lib, g, at, tile=5, split=1)
-    assert h.plan()["phase_bound"] > 5e-6
-    for tile in (7, 8):
+    assert h.plan()["tile"] not in (5, 6, 9, 10, 11)
+    for tile in (5, 9, 10, 11):
+        f, h = run(lib, g, at, tile=tile, split=1)
+        assert h.plan()["phase_bound"] > 5e-6
+    for tile in (7, 8, 12):
         f, h = run(lib, g, at, tile=tile, split=2)
         assert h.plan()["phase_bound"] <= 5e-6
         assert rel_l2(f, ref) <= SPLIT_RTOL and rel_l2(intensity(f), intensity(ref)) <= SPLIT_RTOL
-    for tile in (5, 7, 8):
+    for tile in (5, 7, 8, 9, 12):
         with pytest.raises(lib.IdisimError):
             run(lib, g, at, tile=tile, split=1, summode=1)
     # the same detector at 0.5 m: eligible, and within the bound of the oracle
@@ -316,12 +319,12 @@ def test_pair_tiles_tiny_detector_and_shots(lib, oracle_mod):
         for n in (1, 2, 300):
             at, _ = sc_ball(rng, n, 3e-4, 0.01)
             ref = oracle_mod.field_c(g, at)
-            for tile in (5, 6, 7, 8):
+            for tile in (5, 6, 7, 8, 9, 10, 11, 12):
                 f, h = run(lib, g, at, tile=tile, split=1)
                 assert f.shape == (n0, n1)
                 assert rel_l2(f, ref) <= SPLIT_RTOL, (n0, n1, n, tile)
             h = make_handle(lib, g, lib.PREC_FP32SPLIT)
-            h.set_option(lib.OPT_TILE, 8 if n == 300 else 7)
+            h.set_option(lib.OPT_TILE, {1: 9, 2: 12, 300: 8}[n])
             offsets = np.array([0, n, n, 2 * n], dtype=np.int64)
             out = h.forward_batched(torch.from_numpy(np.concatenate([at, at])).cuda(), offsets).cpu().numpy()
             assert np.count_nonzero(out[1]) == 0

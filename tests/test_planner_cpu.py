@@ -131,3 +131,73 @@ def test_block_pair_centre_bound(args, slack):
     true, bound = _block_true_coef(g), _block_bound_coef(g)
     assert true <= bound, (true, bound)
     assert bound <= 1.5 * slack * true, (true, bound)
+
+
+def _ref_bound_coef(g):
+    """1.05 C(s) krev pix^2 |a| |o| / R^2 with |a| = sqrt(2)/2, |o| = 7 sqrt(2) pixels, C(s) = 1 + (2 + sqrt 2) s
+    + 6 s^2, and s, R over the detector padded to whole 32 x 32 tiles (plus one pixel before the first)."""
+    x0 = (-1 - g.n0 / 2) * g.pixelsize + g.detoffset[0]
+    x1 = (32 * ((g.n0 + 31) // 32) - g.n0 / 2) * g.pixelsize + g.detoffset[0]
+    y0 = (-1 - g.n1 / 2) * g.pixelsize + g.detoffset[1]
+    y1 = (32 * ((g.n1 + 31) // 32) - g.n1 / 2) * g.pixelsize + g.detoffset[1]
+    nx, ny = min(max(0.0, x0), x1), min(max(0.0, y0), y1)
+    xs = np.arange(g.n0, dtype=np.float64)
+    ys = np.arange(g.n1, dtype=np.float64)
+    detx = (xs[:, None] - g.n0 / 2) * g.pixelsize + g.detoffset[0]
+    dety = (ys[None, :] - g.n1 / 2) * g.pixelsize + g.detoffset[1]
+    rmin = min(float(np.sqrt(detx**2 + dety**2 + g.detz**2).min()), float(np.sqrt(nx**2 + ny**2 + g.detz**2)))
+    s = min(1.0, max(abs(x0), abs(x1), abs(y0), abs(y1)) / rmin)
+    return (1.05 * (1 + (2 + np.sqrt(2)) * s + 6 * s * s) * g.k / (2 * np.pi) * g.pixelsize**2
+            * (np.sqrt(2) / 2) * (7 * np.sqrt(2)) / rmin**2)
+
+
+def _ref_true_coef(g):
+    """max over the 16 x 16 pixel wave blocks of 32 x 32 tiles and their quads that hold a detector pixel of
+    krev |d_k(quad) - D_k(block centre)|, d_0 = (dq(1,1) - dq(0,0)) / 2, d_1 = (dq(0,1) - dq(1,0)) / 2."""
+    X, Y = 32 * ((g.n0 + 31) // 32), 32 * ((g.n1 + 31) // 32)
+    u = _unit_dirs(g, np.arange(X + 1, dtype=np.float64), np.arange(Y + 1, dtype=np.float64))
+    d0 = 0.5 * (u[:, 1:, 1:] - u[:, :-1, :-1])  # quad with lower corner (x, y), shape (3, X, Y)
+    d1 = 0.5 * (u[:, :-1, 1:] - u[:, 1:, :-1])
+    worst = 0.0
+    for bx in range(0, X, 16):
+        for by in range(0, Y, 16):
+            qx = np.arange(bx, bx + 16, 2)
+            qy = np.arange(by, by + 16, 2)
+            qx, qy = qx[qx < g.n0], qy[qy < g.n1]
+            if qx.size == 0 or qy.size == 0:
+                continue
+            for d in (d0, d1):
+                D = d[:, bx + 7, by + 7]
+                diff = d[:, qx[:, None], qy[None, :]] - D[:, None, None]
+                worst = max(worst, float(np.linalg.norm(diff, axis=0).max()))
+    return g.k / (2 * np.pi) * worst
+
+
+@pytest.mark.parametrize("args,slack", CASES)
+def test_wave_reference_bound(args, slack):
+    """The bound of |d_k - D_k| that tile variants 9..12 rotate across (a lane's pair half difference
+    against its wave block's centre), against the exact maximum over every block of the detector."""
+    from oracle import Geometry
+
+    g = Geometry(args["n0"], args["n1"], args["pixelsize"], args["detz"], K_UM,
+                 detoffset=args.get("detoffset", (0.0, 0.0)), detangles=args.get("detangles", (0.0, 0.0)))
+    true, bound = _ref_true_coef(g), _ref_bound_coef(g)
+    assert true <= bound, (true, bound)
+    assert bound <= 1.5 * slack * true, (true, bound)
+
+
+def test_wave_reference_bound_catches_a_wrong_centre():
+    """Taking D_k at a corner of the wave block instead of its centre doubles the offset the lanes
+    rotate across, and the exact maximum then exceeds the bound (the bound is not vacuous)."""
+    from oracle import Geometry
+
+    g = Geometry(512, 512, 75.0, 2e5, K_UM)
+    X = 512
+    u = _unit_dirs(g, np.arange(X + 1, dtype=np.float64), np.arange(X + 1, dtype=np.float64))
+    d0 = 0.5 * (u[:, 1:, 1:] - u[:, :-1, :-1])
+    worst = 0.0
+    for bx in range(0, X, 16):
+        for by in range(0, X, 16):
+            diff = d0[:, bx:bx + 16:2, by:by + 16:2] - d0[:, bx, by][:, None, None]
+            worst = max(worst, float(np.linalg.norm(diff, axis=0).max()))
+    assert g.k / (2 * np.pi) * worst > _ref_bound_coef(g)
