@@ -42,6 +42,9 @@ FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector (published)
 OPS_PER_PAIR_NF = 23       # near field, unit amplitude: 3 sub, 3 mul + 2 add, sqrt, 4 (phase), cos, sin, 8 (Kahan)
 OPS_PER_PAIR_NF_AMP = 25
 OPS_PER_PAIR_ADJ = 33
+# VALU instructions (of them transcendental) issued per pair by the tile variants of the single-image
+# forward, from rocprofv3 SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 on the 512x512 image
+ISSUED_PER_PAIR = {2: {"valu": 7.28, "trans": 2.00}, 5: {"valu": 6.03, "trans": 1.50}, 9: {"valu": 6.21, "trans": 0.52}}
 
 
 def log(*a):
@@ -184,7 +187,8 @@ def bench_single(args):
     traffic = args.traffic
     if traffic is None:
         # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE for this plan
-        for name in ("fwd_pair_sc100k_512px_traffic.json", "fwd_split_sc100k_512px_traffic.json"):
+        for name in ("fwd_ref_sc100k_512px_traffic.json", "fwd_pair_sc100k_512px_traffic.json",
+                     "fwd_split_sc100k_512px_traffic.json"):
             try:
                 with open(os.path.join(ROOT, "profiles", name)) as f:
                     rec = json.load(f)
@@ -220,6 +224,10 @@ def bench_single(args):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "ops_per_pair": OPS_PER_PAIR_NF, "kernel_ms": split_ms,
+                     # ops_per_pair counts the reference's per-pair formula; the mirror-pair tiles share
+                     # sincos and phase work across pixels, so they issue fewer instructions per pair than
+                     # that count and frac can exceed 1
+                     "issued_per_pair": ISSUED_PER_PAIR.get(plan["tile"]),
                      "fp64_frac": pairs * OPS_PER_PAIR_NF / (float(np.mean(ms64)) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
     }
