@@ -59,6 +59,31 @@ def test_golden_split(lib, golden):
             assert plan["phase_bound"] <= 5e-6
 
 
+def test_golden_split_every_tile_within_its_bound(lib, golden):
+    """Every FP32-split tile variant forced on every golden case: the field error stays within
+    2 pi x the variant's own a-priori bound, also where that bound is above the tolerance (a variant
+    the planner would not pick); variants a mode does not allow (mirror pairs without the W fold)
+    refuse with an error."""
+    checked = 0
+    for case in golden:
+        g = golden_geometry(case)
+        atoms = np.array(case["atoms"])
+        amp = None if case["amp"] is None else np.array(case["amp"])
+        ref = golden_field(case)
+        scale = golden_term_scale(case)
+        for tile in range(13):
+            try:
+                f, h = run(lib, g, atoms, amp, tile=tile, split=1)
+            except lib.IdisimError:
+                continue
+            plan = h.plan()
+            assert plan["tile"] == tile and plan["fallback"] == 0
+            err = np.abs(f - ref).max() / scale
+            assert err <= 2 * np.pi * plan["phase_bound"] + 2e-7, (case["name"], tile, err, plan)
+            checked += 1
+    assert checked >= 8 * len(golden)
+
+
 def _c1(rng):
     from oracle import Geometry
 
