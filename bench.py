@@ -184,6 +184,17 @@ def bench_single(args):
                          f"({len(rows) * n1 * len(atoms):.3e} pairs, {dt:.1f} s) of the same image; "
                          f"FP64 C/OpenMP restatement of cpu.py:30-107, {cpu_model()}"}
     achieved = pairs * OPS_PER_PAIR_NF / (split_ms * 1e-3) / 1e12
+    # issue ceiling of the instruction mix the planned tile issues, from the v_fma_f32 and
+    # v_sin_f32 / v_cos_f32 rates measured on this device
+    fma_rate, trans_rate = _lib.issue_rates()
+    mix = ISSUED_PER_PAIR.get(plan["tile"])
+    issue = {"fma_per_s": fma_rate, "trans_per_s": trans_rate}
+    if mix is not None:
+        # FMA-class and transcendental issue do not overlap
+        ceiling = 1.0 / ((mix["valu"] - mix["trans"]) / fma_rate + mix["trans"] / trans_rate)
+        issue.update({"valu_per_pair": mix["valu"], "trans_per_pair": mix["trans"],
+                      "ceiling_pairs_per_s": ceiling, "frac": pairs / (split_ms * 1e-3) / ceiling})
+    log("issue rates", issue)
     traffic = args.traffic
     if traffic is None:
         # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE for this plan
@@ -227,7 +238,7 @@ def bench_single(args):
                      # ops_per_pair counts the reference's per-pair formula; the mirror-pair tiles share
                      # sincos and phase work across pixels, so they issue fewer instructions per pair than
                      # that count and frac can exceed 1
-                     "issued_per_pair": ISSUED_PER_PAIR.get(plan["tile"]),
+                     "issue": issue,
                      "fp64_frac": pairs * OPS_PER_PAIR_NF / (float(np.mean(ms64)) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
     }

@@ -99,12 +99,14 @@ def load():
     lib.idisim_corr_accumulate.argtypes = [vp, vp, i64, i64, vp, vp, vp]
     lib.idisim_qcorr.argtypes = [vp, vp, i64, i64, dbl, dbl, dbl, i64, i64, vp, vp]
     lib.idisim_generate_balls.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl, i64, vp, vp, ip, ip, vp, vp]
+    lib.idisim_issue_rates.argtypes = [ip] + [ctypes.POINTER(ctypes.c_double)] * 2
     lib.idisim_generate_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
                  "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
                  "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss",
-                 "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward", "idisim_generate_choice"):
+                 "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward", "idisim_generate_choice",
+                 "idisim_issue_rates"):
         getattr(lib, name).restype = ip
     if lib.idisim_version() != 1:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
@@ -267,6 +269,17 @@ class ChoiceSampler:
                                                int(bool(phases)), _ptr(out), _ptr(self.workspace), ctypes.byref(nb),
                                                _stream()))
         return out
+
+
+def issue_rates(device=None):
+    """(v_fma_f32, v_sin_f32/v_cos_f32) lane operations per second measured on `device` (default: the
+    current one): the issue rates that bound the FP32-split kernels."""
+    torch = _require_cuda()
+    lib = load()
+    dev = torch.cuda.current_device() if device is None else int(device)
+    f, t = ctypes.c_double(), ctypes.c_double()
+    check(lib.idisim_issue_rates(dev, ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 def chargesharing(counts, out, sigma, gain_noise, seed):

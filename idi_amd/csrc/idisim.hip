@@ -669,6 +669,54 @@ extern "C" {
 
 int idisim_version(void) { return IDISIM_VERSION; }
 
+int idisim_issue_rates(int device, double* fma_per_s, double* trans_per_s)
+{
+    g_err.clear();
+    if (!fma_per_s || !trans_per_s) return fail(IDISIM_E_INVALID, "idisim_issue_rates: NULL output");
+    int ndev = 0;
+    IDISIM_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(IDISIM_E_INVALID, "idisim_issue_rates: no such device");
+    DeviceGuard guard(device);
+    int cus = 0;
+    IDISIM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const int blocks = 16 * std::max(cus, 1);  // two rounds of 32 waves per CU
+    struct Res {
+        float* out = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Res()
+        {
+            if (st) (void)hipStreamSynchronize(st);
+            if (out) (void)hipFree(out);
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } r;
+    IDISIM_HIP(hipMalloc(&r.out, (size_t)blocks * 256 * sizeof(float)));
+    IDISIM_HIP(hipStreamCreate(&r.st));
+    IDISIM_HIP(hipEventCreate(&r.a));
+    IDISIM_HIP(hipEventCreate(&r.b));
+    double rate[2] = {0.0, 0.0};
+    for (int kind = 0; kind < 2; ++kind) {
+        const int iters = kind == 0 ? 1024 : 512;  // ~1 ms per launch on 256 CUs
+        const double ops = (double)blocks * 256.0 * iters * (kind == 0 ? 64.0 : 32.0);
+        IDISIM_HIP(launch_issue_probe(kind, blocks, iters, r.out, r.st));  // warm-up
+        for (int rep = 0; rep < 3; ++rep) {
+            IDISIM_HIP(hipEventRecord(r.a, r.st));
+            IDISIM_HIP(launch_issue_probe(kind, blocks, iters, r.out, r.st));
+            IDISIM_HIP(hipEventRecord(r.b, r.st));
+            IDISIM_HIP(hipEventSynchronize(r.b));
+            float ms = 0.f;
+            IDISIM_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            if (ms > 0.f) rate[kind] = std::max(rate[kind], ops / (ms * 1e-3));
+        }
+    }
+    *fma_per_s = rate[0];
+    *trans_per_s = rate[1];
+    return IDISIM_OK;
+}
+
 const char* idisim_last_error(void) { return g_err.c_str(); }
 
 int idisim_create(idisim_handle** out, const idisim_geometry* geom, int precision, int device)

@@ -59,6 +59,8 @@ hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& 
                           float* grad_pos, float* grad_phase, float* grad_amp, hipStream_t st);
 
 hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out, hipStream_t st);
+// issue-rate probe (kind 0: v_fma_f32, 1: v_sin_f32 / v_cos_f32); out: blocks x 256 floats
+hipError_t launch_issue_probe(int kind, int blocks, int iters, float* out, hipStream_t st);
 hipError_t launch_zero(void* out, size_t bytes, hipStream_t st);  // bytes: a multiple of 4
 
 // Emitters in a ball / random phases from Philox4x32-10 (generate.hip).

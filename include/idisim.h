@@ -267,6 +267,15 @@ IDISIM_API int idisim_corr_accumulate(const float *corr, const float *maskcorr, 
 IDISIM_API int idisim_qcorr(const float *zd, const float *img, int64_t nr, int64_t nc, double xcenter,
                             double ycenter, double z, int64_t qmax, int64_t maxdqz, double *out, void *stream);
 
+/* Issue rates of the instructions that bound the FP32-split kernels, measured on
+   `device`: v_fma_f32 and v_sin_f32 / v_cos_f32 lane operations per second, each
+   alone.  The two do not overlap in issue, so work that takes f FMA-class and t
+   transcendental lane operations per unit runs at most 1 / (f / fma + t / trans)
+   units per second.  No counterpart in the reference; for callers that
+   state a kernel's issue ceiling (the benchmark, autotuning across devices).
+   Synchronous, a few milliseconds, on a stream of its own.                      */
+IDISIM_API int idisim_issue_rates(int device, double *fma_per_s, double *trans_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,7 @@ def test_exports_every_declared_symbol(lib):
     from idi_amd import _lib
 
     names = _lib.declared_functions()
-    assert len(names) == 18
+    assert len(names) == 19
     for n in names:
         assert hasattr(lib, n), n
     assert lib.idisim_version() == 1
@@ -63,6 +63,7 @@ def test_argument_errors_without_device(lib):
     assert lib.idisim_set_option(None, 1, 1) == -1
     assert lib.idisim_get_plan(None, None) == -1
     assert lib.idisim_extent(None, 5, None, None) == -1
+    assert lib.idisim_issue_rates(0, None, None) == -1
     assert lib.idisim_destroy(None) == 0
 
 

@@ -354,3 +354,14 @@ def test_pair_tiles_tiny_detector_and_shots(lib, oracle_mod):
             out = h.forward_batched(torch.from_numpy(np.concatenate([at, at])).cuda(), offsets).cpu().numpy()
             assert np.count_nonzero(out[1]) == 0
             assert rel_l2(out[0], ref) <= SPLIT_RTOL and rel_l2(out[2], ref) <= SPLIT_RTOL
+
+
+def test_issue_rates(lib):
+    """The v_fma_f32 and v_sin_f32 / v_cos_f32 issue rates the benchmark states the forward kernels'
+    ceiling with: positive, transcendentals slower than FMA, and of the order of one instruction per
+    SIMD lane per clock (between 1e12 and 1e15 lane operations per second on any CDNA device)."""
+    fma, trans = lib.issue_rates()
+    print("issue rates", fma, trans)
+    assert 1e12 < trans < fma < 1e15
+    with pytest.raises(lib.IdisimError):
+        lib.issue_rates(999)
