@@ -226,7 +226,8 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
 // e^{i t}, t = 2 pi (d_k - D_k) . r' (|t| a few 1e-3 rad), taken to first order
 // (1 + i t; REF 1) or second order (1 + i t - t^2 / 2; REF 2).  Per quad and
 // emitter that is 2 transcendentals and 21 (REF 1) or 27 (REF 2) other VALU where
-// REF 0 spends 6 and 17.  The host planner bounds t^2 / 2 (REF 1) or |t|^3 / 6
+// REF 0 spends 6 and 17 (plus 2 multiplies by the amplitude, which with a reference
+// is folded into E_k instead).  The host planner bounds t^2 / 2 (REF 1) or |t|^3 / 6
 // (REF 2).  sD holds the four waves' D_k (FP32, 24 floats) and sE their phasors,
 // (cos, sin) of both pairs per float4, with REF 2 a second float4 scaled by sqrt 2.
 template <int REF>
@@ -336,8 +337,9 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                         float dl = D[0] * r.x;
                         dl = fmaf(D[1], r.y, dl);
                         dl = fmaf(D[2], r.z, dl);
-                        es[k] = __builtin_amdgcn_sinf(dl);
-                        ec[k] = __builtin_amdgcn_cosf(dl);
+                        // the amplitude rides on the phasor: the lanes' products are linear in it
+                        es[k] = HAS_AMP ? r.amp * __builtin_amdgcn_sinf(dl) : __builtin_amdgcn_sinf(dl);
+                        ec[k] = HAS_AMP ? r.amp * __builtin_amdgcn_cosf(dl) : __builtin_amdgcn_cosf(dl);
                     }
                     sE[(REF * w) * kChunk + tid] = make_float4(ec[0], es[0], ec[1], es[1]);
                     if (REF == 2) {
@@ -360,7 +362,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
 #pragma unroll 2
         for (int e = 0; e < cnt; ++e) {
             const float4 r0 = s0[e];
-            const float am = HAS_AMP ? s1[e].y : 1.f;
+            const float am = (HAS_AMP && REF == 0) ? s1[e].y : 1.f;
             float4 E = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E;
             if (REF != 0) E = eref[e];
             if (REF == 2) E2 = eref[kChunk + e];
@@ -371,7 +373,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                 t = fmaf(mc[q][2], r0.z, t);
                 float sc = __builtin_amdgcn_sinf(t);
                 float cc = __builtin_amdgcn_cosf(t);
-                if (HAS_AMP) {
+                if (HAS_AMP && REF == 0) {  // with a reference the amplitude is in E
                     sc *= am;
                     cc *= am;
                 }
