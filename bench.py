@@ -45,6 +45,20 @@ OPS_PER_PAIR_ADJ = 33
 # VALU instructions (of them transcendental) issued per pair by the tile variants of the single-image
 # forward, from rocprofv3 SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 on the 512x512 image
 ISSUED_PER_PAIR = {2: {"valu": 7.28, "trans": 2.00}, 5: {"valu": 6.03, "trans": 1.50}, 9: {"valu": 6.21, "trans": 0.52}}
+# the same for the persistent batched kernel on the 256-shot workload
+ISSUED_PER_PAIR_BATCHED = {8: {"valu": 7.16, "trans": 1.25}, 12: {"valu": 8.95, "trans": 0.52}}
+
+
+def issue_fields(lib, mix, pairs_per_s):
+    """Issue ceiling of an instruction mix (VALU and transcendental per pair) from the v_fma_f32 and
+    v_sin_f32 / v_cos_f32 rates measured on this device (the two do not overlap in issue)."""
+    fma_rate, trans_rate = lib.issue_rates()
+    issue = {"fma_per_s": fma_rate, "trans_per_s": trans_rate}
+    if mix is not None:
+        ceiling = 1.0 / ((mix["valu"] - mix["trans"]) / fma_rate + mix["trans"] / trans_rate)
+        issue.update({"valu_per_pair": mix["valu"], "trans_per_pair": mix["trans"],
+                      "ceiling_pairs_per_s": ceiling, "frac": pairs_per_s / ceiling})
+    return issue
 
 
 def log(*a):
@@ -186,14 +200,7 @@ def bench_single(args):
     achieved = pairs * OPS_PER_PAIR_NF / (split_ms * 1e-3) / 1e12
     # issue ceiling of the instruction mix the planned tile issues, from the v_fma_f32 and
     # v_sin_f32 / v_cos_f32 rates measured on this device
-    fma_rate, trans_rate = _lib.issue_rates()
-    mix = ISSUED_PER_PAIR.get(plan["tile"])
-    issue = {"fma_per_s": fma_rate, "trans_per_s": trans_rate}
-    if mix is not None:
-        # FMA-class and transcendental issue do not overlap
-        ceiling = 1.0 / ((mix["valu"] - mix["trans"]) / fma_rate + mix["trans"] / trans_rate)
-        issue.update({"valu_per_pair": mix["valu"], "trans_per_pair": mix["trans"],
-                      "ceiling_pairs_per_s": ceiling, "frac": pairs / (split_ms * 1e-3) / ceiling})
+    issue = issue_fields(_lib, ISSUED_PER_PAIR.get(plan["tile"]), pairs / (split_ms * 1e-3))
     log("issue rates", issue)
     traffic = args.traffic
     if traffic is None:
@@ -301,7 +308,9 @@ def bench_batched(args):
             "roofline": {"bound": "valu", "achieved": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
-                         "traffic": None}}
+                         "traffic": None,
+                         "issue": issue_fields(_lib, ISSUED_PER_PAIR_BATCHED.get(h.plan()["tile"]),
+                                               pairs / (np.mean(ms) * 1e-3))}}
 
 
 def three_balls(rng, n_total=1_000_000):
