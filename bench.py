@@ -44,7 +44,8 @@ OPS_PER_PAIR_NF_AMP = 25
 OPS_PER_PAIR_ADJ = 33
 # VALU instructions (of them transcendental) issued per pair by the tile variants of the single-image
 # forward, from rocprofv3 SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 on the 512x512 image
-ISSUED_PER_PAIR = {2: {"valu": 7.28, "trans": 2.00}, 5: {"valu": 6.03, "trans": 1.50}, 9: {"valu": 6.21, "trans": 0.52}}
+ISSUED_PER_PAIR = {2: {"valu": 7.28, "trans": 2.00}, 5: {"valu": 6.03, "trans": 1.50}, 9: {"valu": 6.21, "trans": 0.52},
+                   13: {"valu": 5.75, "trans": 0.52}}
 # the same for the persistent batched kernel on the 256-shot workload
 ISSUED_PER_PAIR_BATCHED = {8: {"valu": 7.16, "trans": 1.25}, 12: {"valu": 8.95, "trans": 0.52}}
 
@@ -158,6 +159,8 @@ def bench_single(args):
     ext = float(np.linalg.norm(atoms[:, :3], axis=1).max())
 
     hs = _lib.Handle(n0, n1, 75.0, 2e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    if args.tile is not None:  # pin a tile variant (profiling one kernel); the planner's split
+        hs.set_option(_lib.OPT_TILE, args.tile)
     out = hs.forward(d, None, None, _lib.OUT_FIELD, ext)  # autotune + first run
     torch.cuda.synchronize()
     plan = hs.plan()
@@ -205,8 +208,8 @@ def bench_single(args):
     traffic = args.traffic
     if traffic is None:
         # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE for this plan
-        for name in ("fwd_ref_sc100k_512px_traffic.json", "fwd_pair_sc100k_512px_traffic.json",
-                     "fwd_split_sc100k_512px_traffic.json"):
+        for name in ("fwd_lin_sc100k_512px_traffic.json", "fwd_ref_sc100k_512px_traffic.json",
+                     "fwd_pair_sc100k_512px_traffic.json", "fwd_split_sc100k_512px_traffic.json"):
             try:
                 with open(os.path.join(ROOT, "profiles", name)) as f:
                     rec = json.load(f)
@@ -733,6 +736,8 @@ def main():
     ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep", "e2e", "photons",
                                                          "correlate", "generate"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="single-image workload: run this FP32-split tile variant instead of the planner's choice")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -99,6 +99,43 @@ __host__ __device__ inline Frame pixel_frame(const Geom& g, double fx, double fy
     return make_frame(r);
 }
 
+// Half differences of the 2 x 2 pixel quad with lower corner (x, y) (fractional pixel
+// indices): h_0 = (dq(x+1, y+1) - dq(x, y)) / 2 and h_1 = (dq(x, y+1) - dq(x+1, y)) / 2,
+// dq = -krev (u - uc), the mirror-pair half differences of the pair tiles.
+__host__ __device__ inline void quad_half_diff(const Geom& g, const double uc[3], double x, double y,
+                                               double h[2][3])
+{
+    const Frame f00 = pixel_frame(g, x, y), f11 = pixel_frame(g, x + 1.0, y + 1.0);
+    const Frame f10 = pixel_frame(g, x + 1.0, y), f01 = pixel_frame(g, x, y + 1.0);
+    for (int a = 0; a < 3; ++a) {
+        const double q00 = -g.krev * (f00.u[a] - uc[a]), q11 = -g.krev * (f11.u[a] - uc[a]);
+        const double q10 = -g.krev * (f10.u[a] - uc[a]), q01 = -g.krev * (f01.u[a] - uc[a]);
+        h[0][a] = 0.5 * (q11 - q00);
+        h[1][a] = 0.5 * (q01 - q10);
+    }
+}
+
+// Linear wave reference of tile variant 13 for the 16 x 16 pixel block centred at (px, py):
+// D_k, the half differences of the quad centred there, and their central differences over
+// one pixel, Jx_k = (h_k(+1, 0) - h_k(-1, 0)) / 2 and Jy_k likewise, all as FP32 values.
+__host__ __device__ inline void wave_linear_ref(const Geom& g, const double uc[3], double px, double py,
+                                                float D[2][3], float Jx[2][3], float Jy[2][3])
+{
+    double h0[2][3], hxp[2][3], hxm[2][3], hyp[2][3], hym[2][3];
+    quad_half_diff(g, uc, px - 0.5, py - 0.5, h0);
+    quad_half_diff(g, uc, px + 0.5, py - 0.5, hxp);
+    quad_half_diff(g, uc, px - 1.5, py - 0.5, hxm);
+    quad_half_diff(g, uc, px - 0.5, py + 0.5, hyp);
+    quad_half_diff(g, uc, px - 0.5, py - 1.5, hym);
+    for (int k = 0; k < 2; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            D[k][a] = (float)h0[k][a];
+            Jx[k][a] = (float)(0.5 * (hxp[k][a] - hxm[k][a]));
+            Jy[k][a] = (float)(0.5 * (hyp[k][a] - hym[k][a]));
+        }
+    }
+}
+
 // Per (tile centre, emitter) record of the FP32-split kernels.  Only the large
 // linear phase A needs FP64 (|k_rev u_c . r_j| reaches tens of revolutions and
 // is reduced mod 1); W, sigma and the amplitude factor are small corrections
@@ -173,7 +210,10 @@ struct WorkItem {
 // pixel block, the phasor of delta_k at the block centre is formed once per emitter
 // and wave, and each lane rotates it to its own pair to first (9, 11) or second
 // (10, 12) order.
-constexpr int kNumTiles = 13;
+// 13: as 9, with the rotation angle linear in the quad's offset from the block
+// centre (two per-emitter slopes per pair and wave), so that a lane forms it in two
+// operations instead of three.
+constexpr int kNumTiles = 14;
 constexpr int kTileMfma = 4;
 constexpr int kTilePair1 = 5;
 constexpr int kTilePair2 = 6;
@@ -183,16 +223,21 @@ constexpr int kTileRef1 = 9;
 constexpr int kTileRef2 = 10;
 constexpr int kTileRefC1 = 11;
 constexpr int kTileRefC2 = 12;
-__host__ __device__ constexpr bool tile_pair(int v) { return v >= kTilePair1 && v <= kTileRefC2; }
+constexpr int kTileRefL = 13;
+__host__ __device__ constexpr bool tile_pair(int v) { return v >= kTilePair1 && v <= kTileRefL; }
 __host__ __device__ constexpr bool tile_pair_centred(int v)
 {
     return v == kTilePairC || v == kTileRefC1 || v == kTileRefC2;
 }
-// order of the rotation from the wave reference to the lane's delta_k (0: own sincos)
-__host__ __device__ constexpr int tile_ref_order(int v)
+// kind of wave reference: 0 none (own sincos), 1 first-order rotation, 2 second order,
+// 3 first order with the angle linear in the quad offset
+__host__ __device__ constexpr int tile_ref_kind(int v)
 {
-    return (v == kTileRef1 || v == kTileRefC1) ? 1 : ((v == kTileRef2 || v == kTileRefC2) ? 2 : 0);
+    return (v == kTileRef1 || v == kTileRefC1) ? 1
+           : ((v == kTileRef2 || v == kTileRefC2) ? 2 : (v == kTileRefL ? 3 : 0));
 }
+// order of the rotation from the wave reference to the lane's delta_k (0: own sincos)
+__host__ __device__ constexpr int tile_ref_order(int v) { return tile_ref_kind(v) == 3 ? 1 : tile_ref_kind(v); }
 __host__ __device__ constexpr int pair_quads(int v) { return v == kTilePair2 ? 2 : 1; }
 __host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileMfma ? 2 : 1; }
 __host__ __device__ constexpr int tile_py(int v)

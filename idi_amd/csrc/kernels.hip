@@ -260,7 +260,39 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
 
     // the wave's reference half differences D_k (as the FP32 values the records use)
     double Dref[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-    if (REF != 0) {
+    // REF 3: the quad centre's offset from the block centre (pixels), and the wave's linear
+    // reference (D_k and its slopes, 18 floats per wave) formed by one lane of the wave
+    float ox = 0.f, oy = 0.f;
+    if (REF == 3) {
+        static_assert(REF != 3 || QY == 1, "the linear wave reference has one quad per lane");
+        const int w = tid >> 6, l = tid & 63;
+        ox = (float)(2 * (l >> 3) - 7);
+        oy = (float)(2 * (l & 7) - 7);
+        // lanes 0..4 of each wave form the half differences of the quads centred at the block
+        // centre and one pixel to either side (the terms of wave_linear_ref), staged in the
+        // phasor area of LDS, which no lane reads before the first chunk's records
+        double* sH = reinterpret_cast<double*>(sE);
+        if (l < 5) {
+            const double px = (double)(tx0 + 16 * (w >> 1)) + 7.5;
+            const double py = (double)(ty0 + 16 * (w & 1)) + 7.5;
+            const double sx = l == 1 ? 0.5 : (l == 2 ? -1.5 : -0.5);
+            const double sy = l == 3 ? 0.5 : (l == 4 ? -1.5 : -0.5);
+            double h[2][3];
+            quad_half_diff(g, c.u, px + sx, py + sy, h);
+            for (int k = 0; k < 2; ++k)
+                for (int a = 0; a < 3; ++a) sH[(w * 5 + l) * 6 + k * 3 + a] = h[k][a];
+        }
+        __syncthreads();
+        if (l == 0) {  // the combination of wave_linear_ref, as FP32 values
+            const double* H = sH + w * 30;
+            for (int i = 0; i < 6; ++i) {
+                sD[w * 18 + i] = (float)H[i];
+                sD[w * 18 + 6 + i] = (float)(0.5 * (H[6 + i] - H[12 + i]));
+                sD[w * 18 + 12 + i] = (float)(0.5 * (H[18 + i] - H[24 + i]));
+            }
+        }
+    }
+    if (REF == 1 || REF == 2) {
         const int w = tid >> 6;
         const double px = (double)(tx0 + 16 * (w >> 1)) + 7.5;
         const double py = (double)(ty0 + 16 * QY * (w & 1)) + (8.0 * QY - 0.5);
@@ -274,8 +306,8 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             Dref[0][a] = (double)D0;
             Dref[1][a] = (double)D1;
             if ((tid & 63) == 0) {
-                sD[(2 * w) * 3 + a] = D0;
-                sD[(2 * w + 1) * 3 + a] = D1;
+                sD[w * 18 + a] = D0;
+                sD[w * 18 + 3 + a] = D1;
             }
         }
     }
@@ -315,7 +347,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             }
         }
     }
-    const float4* eref = sE + (REF * (tid >> 6)) * kChunk;  // this wave's phasors
+    const float4* eref = sE + ((REF == 1 ? 1 : 2) * (tid >> 6)) * kChunk;  // this wave's phasors
 
     for (int64_t base = e0; base < e1; base += kChunk) {
         const int cnt = (e1 - base) < (int64_t)kChunk ? (int)(e1 - base) : kChunk;
@@ -327,13 +359,38 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], ea);
             s0[tid] = make_float4(r.x, r.y, r.z, r.A);
             s1[tid] = make_float2(r.W, r.amp);
-            if (REF != 0) {
+            if (REF == 3) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const float* D = sD + w * 18;
+                    float v[12];  // (cos, sin) of both pairs, then the slopes 2 pi Jx_k . r', 2 pi Jy_k . r'
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        float dl = D[k * 3] * r.x;
+                        dl = fmaf(D[k * 3 + 1], r.y, dl);
+                        dl = fmaf(D[k * 3 + 2], r.z, dl);
+                        const float sn = __builtin_amdgcn_sinf(dl), cs = __builtin_amdgcn_cosf(dl);
+                        v[2 * k] = HAS_AMP ? r.amp * cs : cs;
+                        v[2 * k + 1] = HAS_AMP ? r.amp * sn : sn;
+#pragma unroll
+                        for (int s2 = 0; s2 < 2; ++s2) {
+                            const float* J = D + 6 + 6 * s2 + k * 3;
+                            float gl = J[0] * r.x;
+                            gl = fmaf(J[1], r.y, gl);
+                            gl = fmaf(J[2], r.z, gl);
+                            v[4 + 2 * k + s2] = (float)kTwoPi * gl;
+                        }
+                    }
+                    sE[(2 * w) * kChunk + tid] = make_float4(v[0], v[1], v[2], v[3]);
+                    sE[(2 * w + 1) * kChunk + tid] = make_float4(v[4], v[5], v[6], v[7]);
+                }
+            } else if (REF != 0) {
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
                     float ec[2], es[2];
 #pragma unroll
                     for (int k = 0; k < 2; ++k) {
-                        const float* D = sD + (2 * w + k) * 3;
+                        const float* D = sD + w * 18 + k * 3;
                         float dl = D[0] * r.x;
                         dl = fmaf(D[1], r.y, dl);
                         dl = fmaf(D[2], r.z, dl);
@@ -365,7 +422,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             const float am = (HAS_AMP && REF == 0) ? s1[e].y : 1.f;
             float4 E = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E;
             if (REF != 0) E = eref[e];
-            if (REF == 2) E2 = eref[kChunk + e];
+            if (REF >= 2) E2 = eref[kChunk + e];  // REF 2: sqrt 2 E; REF 3: the slopes
 #pragma unroll
             for (int q = 0; q < QY; ++q) {
                 float t = fmaf(mc[q][0], r0.x, r0.w);
@@ -387,16 +444,21 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                 }
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
-                    float dl = d[q][k][0] * r0.x;
-                    dl = fmaf(d[q][k][1], r0.y, dl);
-                    dl = fmaf(d[q][k][2], r0.z, dl);
+                    float dl;
+                    if (REF == 3) {
+                        dl = fmaf(oy, k ? E2.w : E2.y, ox * (k ? E2.z : E2.x));
+                    } else {
+                        dl = d[q][k][0] * r0.x;
+                        dl = fmaf(d[q][k][1], r0.y, dl);
+                        dl = fmaf(d[q][k][2], r0.z, dl);
+                    }
                     float sd, cd;
                     if (REF == 0) {
                         sd = __builtin_amdgcn_sinf(dl);
                         cd = __builtin_amdgcn_cosf(dl);
                     } else {
                         const float ec = k ? E.z : E.x, es = k ? E.w : E.y;
-                        if (REF == 1) {  // (ec + i es)(1 + i t), t = dl
+                        if (REF == 1 || REF == 3) {  // (ec + i es)(1 + i t), t = dl
                             cd = fmaf(-dl, es, ec);
                             sd = fmaf(dl, ec, es);
                         } else {         // (ec + i es)(1 - u^2 + i sqrt2 u), u = dl
@@ -587,8 +649,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
     constexpr int TX = 32, TY = 32 * QY;
     __shared__ float4 s0[kChunk];
     __shared__ float2 s1[kChunk];
-    __shared__ float sD[REF ? 24 : 1];
-    __shared__ float4 sE[REF ? 4 * REF * kChunk : 1];
+    __shared__ float sD[REF ? 72 : 1];
+    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kChunk : 1];
     const int tile = blockIdx.x;
     const int split = blockIdx.y;
     const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
@@ -715,8 +777,8 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* _
     constexpr int TX = 32, TY = 32 * QY;
     __shared__ float4 s0[kChunk];
     __shared__ float2 s1[kChunk];
-    __shared__ float sD[REF ? 24 : 1];
-    __shared__ float4 sE[REF ? 4 * REF * kChunk : 1];
+    __shared__ float sD[REF ? 72 : 1];
+    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kChunk : 1];
     __shared__ int s_item;
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);
@@ -1121,10 +1183,10 @@ static int occupancy_pair_q()
     return nb > 0 ? nb : 1;
 }
 
-// (quads per lane, reference order, centred, amplitudes) -> instantiation; only
+// (quads per lane, reference kind, centred, amplitudes) -> instantiation; only
 // variant 6 has two quads per lane
 #define IDISIM_DISPATCH_PAIR(FN, ...)                                                          \
-    switch ((pair_quads(v) << 4) | (tile_ref_order(v) << 2) | ((int)tile_pair_centred(v) << 1) | (int)a) { \
+    switch ((pair_quads(v) << 4) | (tile_ref_kind(v) << 2) | ((int)tile_pair_centred(v) << 1) | (int)a) { \
         case 16: return FN<1, false, false, 0>(__VA_ARGS__);                                  \
         case 17: return FN<1, true, false, 0>(__VA_ARGS__);                                   \
         case 18: return FN<1, false, true, 0>(__VA_ARGS__);                                   \
@@ -1137,6 +1199,8 @@ static int occupancy_pair_q()
         case 25: return FN<1, true, false, 2>(__VA_ARGS__);                                   \
         case 26: return FN<1, false, true, 2>(__VA_ARGS__);                                   \
         case 27: return FN<1, true, true, 2>(__VA_ARGS__);                                    \
+        case 28: return FN<1, false, false, 3>(__VA_ARGS__);                                  \
+        case 29: return FN<1, true, false, 3>(__VA_ARGS__);                                   \
         case 32: return FN<2, false, false, 0>(__VA_ARGS__);                                  \
         case 33: return FN<2, true, false, 0>(__VA_ARGS__);                                   \
         default: break;                                                                       \

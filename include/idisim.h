@@ -78,8 +78,10 @@ extern "C" {
                                      32x64 px), 9, 10 (as 5) and 11, 12 (as 7)
                                      with the pair phasors rotated from a
                                      per-wave reference to first (9, 11) or
-                                     second (10, 12) order (FP32-split only,
-                                     32x32 px); FP64 handles take 0..3           */
+                                     second (10, 12) order, 13 as 9 with the
+                                     rotation angle linear in the quad offset
+                                     (FP32-split only, 32x32 px); FP64 handles
+                                     take 0..3                                   */
 #define IDISIM_OPT_SPLIT        3 /* force emitter split count (0 = planner)        */
 #define IDISIM_OPT_SUMMODE      4 /* 0 (default): per-chunk partials merged with
                                      Kahan; 1: Kahan on every term (cpu.py:97-105) */

@@ -56,7 +56,7 @@ def test_batched_equals_single_shot(lib, oracle_mod, precision):
         assert rel_l2(out[s], ref) <= (1e-4 if precision == 1 else fp64_tol(g))
 
 
-@pytest.mark.parametrize("tile", [5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("tile", [5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_batched_pair_tiles_equal_single_shot(lib, oracle_mod, tile):
     """Mirror-pair tiles in the persistent batched kernel: bit-identical to the single-shot
     kernel of the same tile (no split), within 1e-4 of the oracle (75 um at 0.2 m, 10 nm balls,

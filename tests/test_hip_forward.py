@@ -71,7 +71,7 @@ def test_golden_split_every_tile_within_its_bound(lib, golden):
         amp = None if case["amp"] is None else np.array(case["amp"])
         ref = golden_field(case)
         scale = golden_term_scale(case)
-        for tile in range(13):
+        for tile in range(14):
             try:
                 f, h = run(lib, g, atoms, amp, tile=tile, split=1)
             except lib.IdisimError:
@@ -203,7 +203,7 @@ def test_empty_and_errors(lib):
     with pytest.raises(ValueError):
         h.forward(torch.zeros((4, 3), dtype=torch.float64, device="cuda"))
     with pytest.raises(lib.IdisimError):
-        h.set_option(lib.OPT_TILE, 13)
+        h.set_option(lib.OPT_TILE, 14)
     with pytest.raises(lib.IdisimError):
         h.set_option(999, 1)
     with pytest.raises(lib.IdisimError):
@@ -273,7 +273,8 @@ def test_autotune_cache_across_extents(lib, oracle_mod):
 def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
     """Mirror-pair tiles (5: one 2 x 2 quad per lane, 6: two, 7: one with a centre per pair, 8: four
     pairs about a 2 x 4 block, 9..12: 5 and 7 with the pair phasors rotated from a per-wave reference
-    to first or second order) on the pixel pitch and distance of
+    to first or second order, 13: as 9 with the rotation angle linear in the quad offset) on the
+    pixel pitch and distance of
     the 100k-site image (75 um at 0.2 m, sites of a 10 nm ball), on an odd detector so that quads
     reach past the last row and column; every emitter split and output kind; a rotated and
     offset detector."""
@@ -287,7 +288,7 @@ def test_pair_tiles_match_oracle(lib, oracle_mod, mode):
     at, _ = sc_ball(rng, 3000, 3e-4, 0.01)
     amp = rng.uniform(0.5, 1.0, len(at)) if "amp" in mode else None
     ref = oracle_mod.field_c(g, at, amp)
-    for tile in (5, 6, 7, 8, 9, 10, 11, 12):
+    for tile in (5, 6, 7, 8, 9, 10, 11, 12, 13):
         for split in (1, 3):
             f, h = run(lib, g, at, amp, tile=tile, split=split)
             p = h.plan()
@@ -313,8 +314,8 @@ def test_pair_tiles_eligibility(lib, oracle_mod):
     at = atoms_ball(rng, 600, 0.03)
     ref = oracle_mod.field_c(g, at)
     _, h = run(lib, g, at)
-    assert h.plan()["tile"] not in (5, 6, 9, 10, 11)
-    for tile in (5, 9, 10, 11):
+    assert h.plan()["tile"] not in (5, 6, 9, 10, 11, 13)
+    for tile in (5, 9, 10, 11, 13):
         f, h = run(lib, g, at, tile=tile, split=1)
         assert h.plan()["phase_bound"] > 5e-6
     for tile in (7, 8, 12):
@@ -344,7 +345,7 @@ def test_pair_tiles_tiny_detector_and_shots(lib, oracle_mod):
         for n in (1, 2, 300):
             at, _ = sc_ball(rng, n, 3e-4, 0.01)
             ref = oracle_mod.field_c(g, at)
-            for tile in (5, 6, 7, 8, 9, 10, 11, 12):
+            for tile in (5, 6, 7, 8, 9, 10, 11, 12, 13):
                 f, h = run(lib, g, at, tile=tile, split=1)
                 assert f.shape == (n0, n1)
                 assert rel_l2(f, ref) <= SPLIT_RTOL, (n0, n1, n, tile)
