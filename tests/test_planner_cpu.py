@@ -201,3 +201,52 @@ def test_wave_reference_bound_catches_a_wrong_centre():
             diff = d0[:, bx:bx + 16:2, by:by + 16:2] - d0[:, bx, by][:, None, None]
             worst = max(worst, float(np.linalg.norm(diff, axis=0).max()))
     assert g.k / (2 * np.pi) * worst > _ref_bound_coef(g)
+
+
+def _lin_residual(g):
+    """max over the 16 x 16 wave blocks of 32 x 32 tiles and their quads holding a detector pixel of
+    krev |d_k(quad) - D_k - ox Jx_k - oy Jy_k|, with D_k the half differences of the quad centred on the
+    block and Jx_k, Jy_k their central differences over one pixel (D, J rounded to float32 as the
+    kernel holds them)."""
+    X, Y = 32 * ((g.n0 + 31) // 32), 32 * ((g.n1 + 31) // 32)
+    u = _unit_dirs(g, np.arange(-2, X + 2, dtype=np.float64), np.arange(-2, Y + 2, dtype=np.float64))
+
+    def h(k, x, y):  # half difference of pair k of the quad with lower corner (x, y); x, y >= -2
+        x, y = x + 2, y + 2
+        if k == 0:
+            return 0.5 * (u[:, x + 1, y + 1] - u[:, x, y])
+        return 0.5 * (u[:, x, y + 1] - u[:, x + 1, y])
+
+    worst = 0.0
+    for bx in range(0, X, 16):
+        for by in range(0, Y, 16):
+            if bx >= g.n0 or by >= g.n1:
+                continue
+            for k in range(2):
+                # block centre bx + 7.5: its quad has lower corner bx + 7
+                D = h(k, bx + 7, by + 7).astype(np.float32).astype(np.float64)
+                Jx = (0.5 * (h(k, bx + 8, by + 7) - h(k, bx + 6, by + 7))).astype(np.float32).astype(np.float64)
+                Jy = (0.5 * (h(k, bx + 7, by + 8) - h(k, bx + 7, by + 6))).astype(np.float32).astype(np.float64)
+                for i in range(8):
+                    for j in range(8):
+                        x, y = bx + 2 * i, by + 2 * j
+                        if x >= g.n0 or y >= g.n1:
+                            continue
+                        r = h(k, x, y) - D - (2 * i - 7) * Jx - (2 * j - 7) * Jy
+                        worst = max(worst, float(np.linalg.norm(r)))
+    return g.k / (2 * np.pi) * worst
+
+
+def test_linear_reference_residual():
+    """The linear wave reference of tile variant 13 on the 512 x 512 image (75 um at 0.2 m): its residual
+    is a small part of the reference offset it replaces (under a third of the |d_k - D_k| bound), and
+    times the 10 nm extent it stays below 1e-6 rev, so the variant qualifies at the 5e-6 rev tolerance;
+    on a coarse detector (100 um at 0.1 m) it does not (above 5e-6 rev for a 30 nm ball)."""
+    from oracle import Geometry
+
+    g = Geometry(512, 512, 75.0, 2e5, K_UM)
+    res = _lin_residual(g)
+    assert res < _ref_bound_coef(g) / 3
+    assert res * 0.01 < 1e-6
+    g2 = Geometry(256, 256, 100.0, 1e5, K_UM)
+    assert _lin_residual(g2) * 0.03 > 5e-6
