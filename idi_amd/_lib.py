@@ -24,6 +24,7 @@ OUT_FIELD, OUT_INTENSITY = 0, 1
 OPT_AUTOTUNE, OPT_TILE, OPT_SPLIT, OPT_SUMMODE, OPT_GRAPH, OPT_TOL_PPB, OPT_FALLBACK = 1, 2, 3, 4, 5, 6, 7
 OPT_TIME_PATH = 8
 E_PRECISION = -4
+ABI_VERSION = 2
 
 
 class GeometryT(ctypes.Structure):
@@ -51,10 +52,12 @@ class PlanT(ctypes.Structure):
         ("fallback", ctypes.c_int32),
         ("phase_bound", ctypes.c_double),
         ("tuned_ms", ctypes.c_double),
+        ("extent_exceeded", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 _lib = None
@@ -108,7 +111,7 @@ def load():
                  "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward", "idisim_generate_choice",
                  "idisim_issue_rates"):
         getattr(lib, name).restype = ip
-    if lib.idisim_version() != 1:
+    if lib.idisim_version() != ABI_VERSION:
         raise RuntimeError("libidisim.so ABI version mismatch; rebuild with `python -m idi_amd.build`")
     _lib = lib
     return lib
@@ -356,6 +359,13 @@ class Handle:
         check(self._lib.idisim_get_plan(self._h, ctypes.byref(p)))
         return p.as_dict()
 
+    def _on_device(self, **tensors):
+        """Every given tensor (None skipped) must live on this handle's GPU: the library reads them
+        as device pointers, and a host or other-device pointer would fault the kernel."""
+        for name, t in tensors.items():
+            if t is not None and (not t.is_cuda or t.device.index != self.device):
+                raise ValueError(f"{name} must be a CUDA tensor on device cuda:{self.device} (got {t.device})")
+
     def forward(self, emit, amp=None, out=None, kind=OUT_FIELD, extent=0.0):
         """emit: (N, 4) float64 CUDA tensor; amp: (N,) float64 or None; returns `out`."""
         import torch
@@ -364,6 +374,7 @@ class Handle:
             raise ValueError("emitters must be a contiguous (N, 4) float64 tensor")
         if amp is not None and (amp.dtype != torch.float64 or amp.shape != (emit.shape[0],) or not amp.is_contiguous()):
             raise ValueError("amplitudes must be a contiguous (N,) float64 tensor")
+        self._on_device(emitters=emit, amplitudes=amp, out=out)
         if out is None:
             out = torch.empty(self.shape, dtype=self.field_dtype if kind == OUT_FIELD else self.real_dtype,
                               device=emit.device)
@@ -384,6 +395,7 @@ class Handle:
             raise ValueError("emitters must be a contiguous (N, 4) float64 tensor")
         if amp is not None and (amp.dtype != torch.float64 or amp.shape != (emit.shape[0],) or not amp.is_contiguous()):
             raise ValueError("amplitudes must be a contiguous (N,) float64 tensor")
+        self._on_device(emitters=emit, amplitudes=amp, out=out)
         if out is None:
             out = torch.empty((nshots,) + self.shape,
                               dtype=self.field_dtype if kind == OUT_FIELD else self.real_dtype, device=emit.device)
@@ -399,10 +411,15 @@ class Handle:
         import torch
 
         n = emit.shape[0]
+        if emit.dtype != torch.float64 or emit.dim() != 2 or emit.shape[1] != 4 or not emit.is_contiguous():
+            raise ValueError("emitters must be a contiguous (N, 4) float64 tensor")
+        if amp is not None and (amp.dtype != torch.float64 or amp.shape != (n,) or not amp.is_contiguous()):
+            raise ValueError("amplitudes must be a contiguous (N,) float64 tensor")
         if field.dtype != torch.complex64 or tuple(field.shape) != self.shape or not field.is_contiguous():
             raise ValueError("field must be the contiguous complex64 (n0, n1) output of forward")
         if dLdI.dtype != torch.float32 or tuple(dLdI.shape) != self.shape or not dLdI.is_contiguous():
             raise ValueError("dLdI must be a contiguous float32 (n0, n1) tensor")
+        self._on_device(emitters=emit, amplitudes=amp, field=field, dLdI=dLdI)
         gpos = torch.empty((n, 3), dtype=torch.float32, device=emit.device)
         gph = torch.empty((n,), dtype=torch.float32, device=emit.device)
         gamp = torch.empty((n,), dtype=torch.float32, device=emit.device) if want_amp else None
@@ -419,6 +436,7 @@ class Handle:
             raise ValueError("emit must be a contiguous (N, 4) float64 CUDA tensor")
         if times.dtype != torch.float64 or tuple(times.shape) != (n,) or not times.is_contiguous():
             raise ValueError("times must be a contiguous (N,) float64 CUDA tensor")
+        self._on_device(emit=emit, times=times, out=out)
         if out is None:
             out = torch.empty(self.shape, dtype=torch.float64, device=emit.device)
         elif out.dtype != torch.float64 or tuple(out.shape) != self.shape or not out.is_contiguous():
@@ -434,3 +452,4 @@ class Handle:
         shape = self.shape if nshots == 1 and out.dim() == 2 else (nshots,) + self.shape
         if out.dtype != want or tuple(out.shape) != shape or not out.is_contiguous() or not out.is_cuda:
             raise ValueError(f"output must be a contiguous CUDA {want} tensor of shape {shape}")
+        self._on_device(out=out)

@@ -21,10 +21,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libidisim.so")
-SOURCES = ["kernels.hip", "kernels_mfma.hip", "adjoint.hip", "generate.hip", "photons.hip", "correlate.hip", "timesim.hip", "choice.hip", "issue.hip", "idisim.hip"]
+SOURCES = ["kernels.hip", "adjoint.hip", "generate.hip", "photons.hip", "correlate.hip", "timesim.hip", "choice.hip", "issue.hip", "idisim.hip"]
 ARCH = "gfx950"
-# MFMA results in VGPRs: the VALU reads tau directly (no v_accvgpr_read per value)
-FILE_FLAGS = {"kernels_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+FILE_FLAGS = {}
 
 
 def hipcc_path():

@@ -1,4 +1,4 @@
-// Device helpers shared by the forward kernels (kernels.hip, kernels_mfma.hip).
+// Device helpers shared by the forward kernels (kernels.hip).
 #pragma once
 #include "kernels.hpp"
 

@@ -43,6 +43,11 @@ struct Geom {
     int n0, n1;
     int rot, nf, scale, secondorder;
     int foldw;   // set per launch: add W_cj into A_cj (lam_p ~ 1 within the bound)
+    // Extent check of the FP32-split record stage: an emitter with |r_j|^2 > ext2 (the
+    // square of the extent the error planner assumed) sets *viol, a flag in host-visible
+    // memory that the host reads on its next call.  viol == nullptr: no check.
+    double ext2;
+    unsigned int* viol;
 };
 
 struct Frame {
@@ -151,6 +156,7 @@ __host__ __device__ inline SplitRec make_split_rec(const Geom& g, const Frame& c
                                                    double ey, double ez, double ephi, double ea)
 {
 #pragma clang fp contract(off)
+    if (g.viol != nullptr && ex * ex + ey * ey + ez * ez > g.ext2) *g.viol = 1u;
     const double ucr = c.u[0] * ex + c.u[1] * ey + c.u[2] * ez;
     const double tau = -g.krev * ucr + ephi * kInv2Pi;
     const float fx = (float)ex, fy = (float)ey, fz = (float)ez;
@@ -198,7 +204,8 @@ struct WorkItem {
 
 // Pixel-tile variants.  0..3: VALU kernels, lanes form a 16 x 16 grid and each
 // lane owns PX x PY pixels (16x16, 16x32, 32x32, 32x64 pixels per workgroup).
-// 4: MFMA kernel (kernels_mfma.hip), 16 x 16 pixels per workgroup.
+// 4: reserved (a matrix-core variant was measured 12% slower than the VALU tiles and
+// is not built: the phase is a nonlinear distance, not a contraction).
 // 5, 6: mirror-pair VALU kernels (one or two 2 x 2 quads per lane; 32x32 and
 // 32x64 pixels per workgroup), which share one sincos per quad and emitter.
 // 7: mirror pairs with a centre per pair (one quad per lane, 32x32 pixels): the
@@ -214,7 +221,7 @@ struct WorkItem {
 // centre (two per-emitter slopes per pair and wave), so that a lane forms it in two
 // operations instead of three.
 constexpr int kNumTiles = 14;
-constexpr int kTileMfma = 4;
+constexpr int kTileNone = 4;  // reserved index, no kernel
 constexpr int kTilePair1 = 5;
 constexpr int kTilePair2 = 6;
 constexpr int kTilePairC = 7;
@@ -239,10 +246,10 @@ __host__ __device__ constexpr int tile_ref_kind(int v)
 // order of the rotation from the wave reference to the lane's delta_k (0: own sincos)
 __host__ __device__ constexpr int tile_ref_order(int v) { return tile_ref_kind(v) == 3 ? 1 : tile_ref_kind(v); }
 __host__ __device__ constexpr int pair_quads(int v) { return v == kTilePair2 ? 2 : 1; }
-__host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileMfma ? 2 : 1; }
+__host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileNone ? 2 : 1; }
 __host__ __device__ constexpr int tile_py(int v)
 {
-    return v == 0 || v == kTileMfma ? 1 : (v == 3 || v == kTilePair2 || v == kTileBlock ? 4 : 2);
+    return v == 0 || v == kTileNone ? 1 : (v == 3 || v == kTilePair2 || v == kTileBlock ? 4 : 2);
 }
 __host__ __device__ constexpr int tile_rows(int v) { return 16 * tile_px(v); }
 __host__ __device__ constexpr int tile_cols(int v) { return 16 * tile_py(v); }

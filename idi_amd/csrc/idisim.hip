@@ -96,6 +96,11 @@ struct idisim_handle {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     unsigned long long* d_ext = nullptr;
+    // extent check: the FP32-split record stage sets the flag (pinned host memory, read by
+    // the host without a synchronisation) when an emitter lies beyond the planner's extent
+    unsigned int* h_viol = nullptr;   // host view
+    unsigned int* d_viol = nullptr;   // device view of the same word
+    int viol_seen = 0;                // the flag was found set (reported by idisim_get_plan)
     // batched work items
     WorkItem* d_items = nullptr;
     size_t items_cap = 0;
@@ -393,7 +398,7 @@ std::vector<int> eligible_tiles(const idisim_handle* h, double rho, double* best
     std::vector<int> out;
     *best_bound = std::numeric_limits<double>::infinity();
     for (int v = 0; v < kNumTiles; ++v) {
-        if ((v == kTileMfma || tile_pair(v)) && h->summode != 0) continue;
+        if (v == kTileNone || (tile_pair(v) && h->summode != 0)) continue;
         const double b = plan_bound(h, v, rho);
         *best_bound = std::min(*best_bound, b);
         if (b <= h->tol_rev) out.push_back(v);
@@ -415,6 +420,23 @@ struct Timer {
     }
 };
 
+// Geometry of an FP32-split launch: the W fold decided for this tile and the call's extent,
+// and the extent check of the record stage (|r_j| beyond the extent the bound assumed).
+Geom split_geom(const idisim_handle* h, int tile, bool* w)
+{
+    Geom gl = h->g;
+    *w = gl.nf || gl.secondorder;
+    if (*w && fold_error(h, tile, h->rho) <= 0.1 * h->tol_rev) {
+        gl.foldw = 1;
+        *w = false;
+    }
+    // a relative margin far above the FP64 rounding of |r_j|^2 and of the caller's extent
+    const double e = h->rho * (1.0 + 1e-9);
+    gl.ext2 = e * e;
+    gl.viol = h->d_viol;
+    return gl;
+}
+
 // ------------------------------ single shot --------------------------------
 struct FwdCall {
     const double* emit;
@@ -427,27 +449,32 @@ struct FwdCall {
     bool out_f32;     // FP64 kernels writing a complex64 / float32 output (fallback)
 };
 
+// Device scratch of `bytes` for one call: the handle's buffer, grown on demand.
+int call_scratch(idisim_handle* h, const FwdCall&, size_t bytes, void** p)
+{
+    int rc = ensure_scratch(h, bytes);
+    if (rc) return rc;
+    *p = h->scratch;
+    return IDISIM_OK;
+}
+
 int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
 {
     const Geom& g = h->g;
     const size_t npix = (size_t)g.n0 * g.n1;
     if (c.split_path) {
-        Geom gl = g;
-        bool w = g.nf || g.secondorder;
-        if (w && fold_error(h, tile, h->rho) <= 0.1 * h->tol_rev) {
-            gl.foldw = 1;
-            w = false;
-        }
+        bool w = false;
+        const Geom gl = split_geom(h, tile, &w);
         const bool a = (c.amp != nullptr) || (g.nf && g.scale);
         float4* partial = nullptr;
         if (splits > 1) {
-            int rc = ensure_scratch(h, (size_t)splits * npix * sizeof(float4));
+            void* p = nullptr;
+            int rc = call_scratch(h, c, (size_t)splits * npix * sizeof(float4), &p);
             if (rc) return rc;
-            partial = static_cast<float4*>(h->scratch);
+            partial = static_cast<float4*>(p);
         }
-        if (tile == kTileMfma) {
-            if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the MFMA tile has no per-term Kahan mode");
-            IDISIM_HIP(launch_fwd_mfma(gl, c.emit, c.amp, c.n, splits, partial, c.out, c.kind, c.st));
+        if (tile == kTileNone) {
+            return fail(IDISIM_E_UNSUPPORTED, "tile variant 4 has no kernel");
         } else if (tile_pair(tile)) {
             if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles have no per-term Kahan mode");
             if (w) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles need the curvature folded into A");
@@ -463,9 +490,10 @@ int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
     // handle) always goes through the finalize kernel, which converts.
     double4* partial = nullptr;
     if (splits > 1 || c.out_f32) {
-        int rc = ensure_scratch(h, (size_t)splits * npix * sizeof(double4));
+        void* p = nullptr;
+        int rc = call_scratch(h, c, (size_t)splits * npix * sizeof(double4), &p);
         if (rc) return rc;
-        partial = static_cast<double4*>(h->scratch);
+        partial = static_cast<double4*>(p);
     }
     IDISIM_HIP(launch_fwd_fp64(tile, g, c.emit, c.amp, c.n, splits, partial, c.out, c.kind, c.st));
     if (partial != nullptr) {
@@ -552,7 +580,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     } else {
         // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -626,17 +654,11 @@ int launch_batched(idisim_handle* h, const BatchCall& c, int tile, int blocks_pe
     const Geom& g = h->g;
     const int nblocks = std::max(1, blocks_per_cu * h->num_cus);
     if (c.split_path) {
-        Geom gl = g;
-        bool w = g.nf || g.secondorder;
-        if (w && fold_error(h, tile, h->rho) <= 0.1 * h->tol_rev) {
-            gl.foldw = 1;
-            w = false;
-        }
+        bool w = false;
+        const Geom gl = split_geom(h, tile, &w);
         const bool a = (c.amp != nullptr) || (g.nf && g.scale);
-        if (tile == kTileMfma) {
-            if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the MFMA tile has no per-term Kahan mode");
-            IDISIM_HIP(launch_batched_mfma(gl, c.emit, c.amp, h->d_items, h->nitems, h->d_counter, nblocks, c.out,
-                                           c.kind, st));
+        if (tile == kTileNone) {
+            return fail(IDISIM_E_UNSUPPORTED, "tile variant 4 has no kernel");
         } else if (tile_pair(tile)) {
             if (h->summode != 0) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles have no per-term Kahan mode");
             if (w) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles need the curvature folded into A");
@@ -658,7 +680,6 @@ int occupancy(idisim_handle* h, const BatchCall& c, int tile)
     const Geom& g = h->g;
     if (c.split_path) {
         const bool w = (g.nf || g.secondorder) && fold_error(h, tile, h->rho) > 0.1 * h->tol_rev;
-        if (tile == kTileMfma) return occupancy_batched_mfma(!w);
         if (tile_pair(tile)) return occupancy_batched_pair(tile, (c.amp != nullptr) || (g.nf && g.scale));
         return occupancy_batched_split(tile, w, (c.amp != nullptr) || (g.nf && g.scale), h->summode);
     }
@@ -671,14 +692,16 @@ int run_batched(idisim_handle* h, const BatchCall& c, int tile, int bpc)
         h->plan.graph = 0;
         return launch_batched(h, c, tile, bpc, c.st);
     }
-    // the W-fold decision depends on the emitter extent of this call: a graph captured with
-    // another decision must not be replayed
+    // the W-fold decision and the extent check depend on the emitter extent of this call: a
+    // graph captured with another extent must not be replayed
     const bool folded = c.split_path && (h->g.nf || h->g.secondorder) &&
                         fold_error(h, tile, h->rho) <= 0.1 * h->tol_rev;
+    uint64_t rho_bits = 0;
+    std::memcpy(&rho_bits, &h->rho, sizeof(rho_bits));
     std::vector<uintptr_t> key = {(uintptr_t)c.emit,    (uintptr_t)c.amp,       (uintptr_t)c.out,
                                   (uintptr_t)c.kind,    (uintptr_t)h->d_items,  (uintptr_t)h->nitems,
                                   (uintptr_t)tile,      (uintptr_t)bpc,         (uintptr_t)h->summode,
-                                  (uintptr_t)c.split_path, (uintptr_t)folded};
+                                  (uintptr_t)c.split_path, (uintptr_t)folded, (uintptr_t)rho_bits};
     if (key != h->gkey || !h->gexec) {
         if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
         if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -710,7 +733,24 @@ int fill_zero(void* out, size_t bytes, hipStream_t st)
     return IDISIM_OK;
 }
 
+// An earlier FP32-split call of this handle met an emitter beyond the extent its error bound
+// assumed: its kernels set the flag.  Reading it needs no synchronisation, so a kernel that
+// is still running is seen by a later call.  The flag is cleared once reported.
+int check_extent_flag(idisim_handle* h, const char* who)
+{
+    h->viol_seen = 0;
+    if (h->h_viol == nullptr || *reinterpret_cast<volatile unsigned int*>(h->h_viol) == 0u) return IDISIM_OK;
+    *reinterpret_cast<volatile unsigned int*>(h->h_viol) = 0u;
+    h->viol_seen = 1;
+    return fail(IDISIM_E_PRECISION, std::string(who) +
+                                        ": an earlier call of this handle was given an extent below max |r_j| of its "
+                                        "emitters, so the FP32-split error bound of its result did not hold; pass "
+                                        "extent <= 0 to have it measured");
+}
+
 }  // namespace
+
+
 
 // ============================== C ABI =======================================
 extern "C" {
@@ -810,6 +850,18 @@ int idisim_create(idisim_handle** out, const idisim_geometry* geom, int precisio
     g.secondorder = (geom->flags & IDISIM_SECONDORDER) ? 1 : 0;
     planner_tables(h);
     h->plan.precision = precision;
+    if (hipHostMalloc(reinterpret_cast<void**>(&h->h_viol), sizeof(unsigned int),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        h->h_viol = nullptr;
+        delete h;
+        return fail(IDISIM_E_NOMEM, "idisim_create: pinned flag allocation failed");
+    }
+    *h->h_viol = 0u;
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&h->d_viol), h->h_viol, 0) != hipSuccess) {
+        (void)hipHostFree(h->h_viol);
+        delete h;
+        return fail(IDISIM_E_HIP, "idisim_create: hipHostGetDevicePointer failed");
+    }
     *out = h;
     return IDISIM_OK;
 }
@@ -827,6 +879,7 @@ int idisim_destroy(idisim_handle* h)
     if (h->d_items) (void)hipFree(h->d_items);
     if (h->h_items) (void)hipHostFree(h->h_items);
     if (h->d_counter) (void)hipFree(h->d_counter);
+    if (h->h_viol) (void)hipHostFree(h->h_viol);
     delete h;
     return IDISIM_OK;
 }
@@ -838,6 +891,7 @@ int idisim_set_option(idisim_handle* h, int option, int64_t value)
         case IDISIM_OPT_AUTOTUNE: h->autotune = value ? 1 : 0; break;
         case IDISIM_OPT_TILE:
             if (value < -1 || value >= kNumTiles) return fail(IDISIM_E_INVALID, "tile variant out of range");
+            if (value == kTileNone) return fail(IDISIM_E_UNSUPPORTED, "tile variant 4 has no kernel");
             h->force_tile = (int)value;
             break;
         case IDISIM_OPT_SPLIT:
@@ -868,6 +922,8 @@ int idisim_get_plan(const idisim_handle* h, idisim_plan* out)
 {
     if (!h || !out) return fail(IDISIM_E_INVALID, "idisim_get_plan: NULL argument");
     *out = h->plan;
+    out->extent_exceeded =
+        (h->viol_seen || (h->h_viol != nullptr && *reinterpret_cast<volatile unsigned int*>(h->h_viol) != 0u)) ? 1 : 0;
     return IDISIM_OK;
 }
 
@@ -1077,6 +1133,28 @@ static int choose_path(idisim_handle* h, double rho, std::vector<int>* tiles, bo
     return IDISIM_OK;
 }
 
+static int forward_call(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
+                        void* out, int out_kind, hipStream_t st)
+{
+    DeviceGuard guard(h->device);
+    int rc = check_extent_flag(h, "idisim_forward");
+    if (rc) return rc;
+    const size_t npix = (size_t)h->g.n0 * h->g.n1;
+    if (n == 0) return fill_zero(out, npix * out_elem_bytes(h->precision, out_kind), st);
+    double rho = extent;
+    if (!(rho > 0.0)) {
+        rc = measure_extent(h, emitters, n, st, &rho);
+        if (rc) return rc;
+    }
+    std::vector<int> tiles;
+    bool split_path = true;
+    rc = choose_path(h, rho, &tiles, &split_path);
+    if (rc) return rc;
+    FwdCall c{emitters, amplitudes, n, out, out_kind, st, split_path,
+              h->precision == IDISIM_PREC_FP32SPLIT && !split_path};
+    return plan_and_run(h, c, tiles);
+}
+
 int idisim_forward(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
                    void* out, int out_kind, void* stream)
 {
@@ -1087,22 +1165,7 @@ int idisim_forward(idisim_handle* h, const double* emitters, const double* ampli
     if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
         return fail(IDISIM_E_INVALID, "idisim_forward: unknown output kind");
     if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward: extent must be finite");
-    DeviceGuard guard(h->device);
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t npix = (size_t)h->g.n0 * h->g.n1;
-    if (n == 0) return fill_zero(out, npix * out_elem_bytes(h->precision, out_kind), st);
-    double rho = extent;
-    if (!(rho > 0.0)) {
-        int rc = measure_extent(h, emitters, n, st, &rho);
-        if (rc) return rc;
-    }
-    std::vector<int> tiles;
-    bool split_path = true;
-    int rc = choose_path(h, rho, &tiles, &split_path);
-    if (rc) return rc;
-    FwdCall c{emitters, amplitudes, n, out, out_kind, st, split_path,
-              h->precision == IDISIM_PREC_FP32SPLIT && !split_path};
-    return plan_and_run(h, c, tiles);
+    return forward_call(h, emitters, amplitudes, n, extent, out, out_kind, static_cast<hipStream_t>(stream));
 }
 
 int idisim_forward_batched(idisim_handle* h, const double* emitters, const double* amplitudes,
@@ -1123,6 +1186,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward_batched: extent must be finite");
     DeviceGuard guard(h->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int rc0 = check_extent_flag(h, "idisim_forward_batched")) return rc0;
     double rho = extent;
     if (!(rho > 0.0)) {
         int rc = measure_extent(h, emitters, total, st, &rho);
@@ -1215,7 +1279,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, kTileMfma, 2, 1, 3, 0})
+        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -1249,6 +1313,7 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
     if (n == 0) return IDISIM_OK;
     DeviceGuard guard(h->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int rc0 = check_extent_flag(h, "idisim_adjoint")) return rc0;
     double rho = extent;
     if (!(rho > 0.0)) {
         int rc = measure_extent(h, emitters, n, st, &rho);
@@ -1260,11 +1325,9 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
     if (bound > h->tol_rev)
         return fail(IDISIM_E_PRECISION, "idisim_adjoint: FP32-split error bound " + std::to_string(bound) +
                                             " rev exceeds the tolerance for this geometry");
-    Geom gl = h->g;
-    const bool curved = gl.nf || gl.secondorder;
-    const bool fold = curved && fold_error(h, 0, rho) <= 0.1 * h->tol_rev;
-    gl.foldw = fold ? 1 : 0;
-    const bool w = curved && !fold;
+    h->rho = rho;
+    bool w = false;
+    const Geom gl = split_geom(h, 0, &w);
     const bool lam = w && !gl.nf;
     const AdjointLayout L = adjoint_layout(h->g, n, h->num_cus, adjoint_occupancy(w, lam, grad_amp != nullptr));
     int rc = ensure_scratch(h, L.bytes);

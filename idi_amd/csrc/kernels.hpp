@@ -27,13 +27,6 @@ hipError_t launch_batched_pair(int v, bool a, const Geom& g, const double* emit,
                                hipStream_t st);
 int occupancy_batched_pair(int v, bool a);
 
-// FP32-split forward with the phase on the matrix cores (tile variant kTileMfma).
-hipError_t launch_fwd_mfma(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
-                           float4* partial, void* out, int kind, hipStream_t st);
-hipError_t launch_batched_mfma(const Geom& g, const double* emit, const double* amp, const WorkItem* items,
-                               int nitems, int* counter, int nblocks, void* out, int kind, hipStream_t st);
-int occupancy_batched_mfma(bool fold);
-
 // Literal FP64 forward.
 hipError_t launch_fwd_fp64(int v, const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
                            double4* partial, void* out, int kind, hipStream_t st);

@@ -6,20 +6,25 @@
 
 ``hip`` is the backend (simulate, simulate_gen, simulate_batched) and ``auto``
 refers to it, as ``idi.simulation.auto`` refers to the CUDA backend when one is
-available (idi/simulation/__init__.py:9-31).  Unlike the reference there is no
-CPU fallback: on a machine without a ROCm GPU the calls raise RuntimeError.
+available (idi/simulation/__init__.py:9-31).  ``cuda`` is the same module under the
+name the reference's scripts call (scripts/idi_sim.py:83), and ``cpu`` keeps the
+reference CPU backend's interface and defaults (far field, FP64, one reused
+output buffer; cpu.py:112-212) on the FP64 GPU kernels.  Unlike the reference
+there is no CPU fallback: on a machine without a ROCm GPU the calls raise
+RuntimeError.
 ``autograd`` holds the torch.autograd.Function with the hand-written adjoint.
 ``cutime`` / ``time`` are the time-dependent simulation (decaying emitters; ``autotime``
 refers to ``cutime``), and
 ``common`` holds photon sampling and detector charge sharing.
 """
-from . import common, cutime, hip, simobj, time  # noqa: F401
+from . import common, cpu, cutime, hip, simobj, time  # noqa: F401
 from .common import chargesharing, randomphotons  # noqa: F401
 
 auto = hip
+cuda = hip  # scripts/idi_sim.py:83 calls sim.cuda.simulate
 autotime = cutime  # idi/simulation/__init__.py:14,30: the time-dependent backend
 
-__all__ = ["hip", "simobj", "auto", "autotime", "autograd", "common", "cutime", "time", "randomphotons",
+__all__ = ["hip", "cpu", "cuda", "simobj", "auto", "autotime", "autograd", "common", "cutime", "time", "randomphotons",
            "chargesharing"]
 
 

@@ -75,7 +75,9 @@ def get_handle(Ndet, pixelsize, detz, k, settings="double", detangles=(0, 0), de
     h = _HANDLES.get(key)
     if h is None:
         if len(_HANDLES) >= _MAX_HANDLES:
-            _HANDLES.pop(next(iter(_HANDLES))).close()
+            # dropped from the cache only: a generator or an autograd graph may still hold it,
+            # and Handle.__del__ frees the library handle once the last reference goes
+            _HANDLES.pop(next(iter(_HANDLES)))
         h = _lib.Handle(n0, n1, pixelsize, detz, k, detoffset=detoffset, rot=rot, nf=f["nf"], scale=f["scale"],
                         secondorder=f["secondorder"],
                         precision=_lib.PREC_FP32SPLIT if f["single"] else _lib.PREC_FP64, device=dev)

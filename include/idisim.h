@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define IDISIM_VERSION 1
+#define IDISIM_VERSION 2
 
 /* geometry flags (settings substrings of cuda.py:53-74 / cpu.py:189-190) */
 #define IDISIM_NF          1u  /* "nf": exact distance (near field)            */
@@ -70,7 +70,7 @@ extern "C" {
 /* options (idisim_set_option) */
 #define IDISIM_OPT_AUTOTUNE     1 /* 1 (default): time candidate plans on first use */
 #define IDISIM_OPT_TILE         2 /* force a tile variant (-1 = planner): 0..3 VALU
-                                     (16x16 .. 32x64 px), 4 MFMA (FP32-split only),
+                                     (16x16 .. 32x64 px), 4 reserved (no kernel),
                                      5, 6 mirror pairs (FP32-split only, 32x32 and
                                      32x64 px), 7 mirror pairs with a centre per
                                      pair (FP32-split only, 32x32 px), 8 four
@@ -119,6 +119,11 @@ typedef struct idisim_plan {
     int32_t fallback;     /* 1 if FP32-split fell back to FP64             */
     double phase_bound;   /* a-priori per-term phase error bound, rev      */
     double tuned_ms;      /* best candidate time measured by the autotuner */
+    int32_t extent_exceeded; /* 1: a finished FP32-split kernel of this handle met an
+                                emitter with |r_j| > the extent of its call, so its
+                                phase_bound did not hold (read after synchronising;
+                                the next call then returns IDISIM_E_PRECISION) */
+    int32_t reserved;
 } idisim_plan;
 
 /* Library ABI version (IDISIM_VERSION). */
@@ -149,7 +154,10 @@ IDISIM_API int idisim_get_plan(const idisim_handle *h, idisim_plan *out);
      emitters:   n x 4 float64 (x, y, z, phi)
      amplitudes: n float64 or NULL (all ones; the reference has no amplitudes)
      extent:     an upper bound of max_j |r_j| (used by the FP32-split error
-                 planner); <= 0 lets the library measure it (synchronises)
+                 planner); <= 0 lets the library measure it (synchronises).  The
+                 FP32-split kernels check it: an emitter beyond it sets a flag, and
+                 the handle's next forward / batched / adjoint call returns
+                 IDISIM_E_PRECISION (idisim_get_plan reports it as extent_exceeded)
      out:        complex64 (FP32SPLIT) / complex128 (FP64) n0 x n1 for FIELD,
                  float32 / float64 n0 x n1 for INTENSITY, row-major, ret[x, y] */
 IDISIM_API int idisim_forward(idisim_handle *h, const double *emitters, const double *amplitudes,

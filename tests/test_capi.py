@@ -26,7 +26,7 @@ def test_exports_every_declared_symbol(lib):
     assert len(names) == 19
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.idisim_version() == 1
+    assert lib.idisim_version() == _lib.ABI_VERSION == 2
 
 
 def test_exports_only_the_c_abi(lib):
@@ -83,3 +83,37 @@ def test_header_declares_reference_counterparts():
     text = open(os.path.join(ROOT, "include", "idisim.h")).read()
     for token in ("sim.cu:141-149", "cpu.py:68-107", "cuda.py:53-86", "no reference counterpart", "No reference"):
         assert token in text, token
+
+
+def test_plan_struct_matches_header():
+    """The ctypes mirror of idisim_plan has the header's size (int32 x 8, double x 2, int32 x 2)."""
+    from idi_amd import _lib
+
+    assert ctypes.sizeof(_lib.PlanT) == 8 * 4 + 2 * 8 + 2 * 4
+    assert [f for f, _ in _lib.PlanT._fields_][-2:] == ["extent_exceeded", "reserved"]
+
+
+def test_inputs_on_another_device_are_rejected_before_any_launch():
+    """Every tensor a Handle method passes to the library must live on the handle's GPU: a host
+    tensor is refused with ValueError before the ctypes call (checked on the CPU with a handle
+    object that holds no library handle, so any call into the library would fail differently)."""
+    import torch
+
+    from idi_amd import _lib
+
+    h = object.__new__(_lib.Handle)
+    h.device, h.shape, h.precision = 0, (4, 4), _lib.PREC_FP32SPLIT
+    h._lib = None  # a library call would raise AttributeError, not ValueError
+    h._h = ctypes.c_void_p()
+    emit = torch.zeros((5, 4), dtype=torch.float64)
+    amp = torch.ones(5, dtype=torch.float64)
+    with pytest.raises(ValueError, match="cuda:0"):
+        h.forward(emit)
+    with pytest.raises(ValueError, match="cuda:0"):
+        h.forward_batched(emit, [0, 2, 5])
+    with pytest.raises(ValueError, match="cuda:0"):
+        h.adjoint(emit, torch.zeros((4, 4), dtype=torch.complex64), torch.zeros((4, 4), dtype=torch.float32), amp)
+    with pytest.raises(ValueError, match="cuda:0"):
+        h.time_forward(emit, torch.zeros(5, dtype=torch.float64), 1.0, 1.0)
+    with pytest.raises(ValueError, match="(?i)cuda"):
+        h._check_out(torch.zeros((4, 4), dtype=torch.complex64), _lib.OUT_FIELD, 1)

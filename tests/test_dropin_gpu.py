@@ -92,3 +92,58 @@ def test_auto_is_hip():
     from idi_amd import simulation
 
     assert simulation.auto is simulation.hip
+
+
+def test_reference_cli_calls_match_oracle(oracle_mod):
+    """The reference CLI's calls (scripts/idi_sim.py:83,85): sim.cuda.simulate(Nimg, obj, Ndet,
+    pixelsize, detz, k) with cuda.py's default settings ("double", far field) and
+    sim.cpu.simulate(...) with cpu.py's ("", far field, FP64); cpu.simulate with "nf scale";
+    cpu.simulate_gen yields one reused complex128 buffer (cpu.py:195,212)."""
+    import oracle
+
+    from idi_amd import simulation as sim
+
+    obj = sim.simobj.sphere(8048.0, 400, 0.05)
+    g = oracle.Geometry(24, 24, 75.0, 1e5, obj.k)
+    for backend in (sim.cuda, sim.cpu):
+        obj.rng = np.random.default_rng(21)
+        res = backend.simulate(2, obj, 24, 75.0, 1e5, obj.k)
+        assert res.shape == (2, 24, 24) and res.dtype == np.complex128
+        obj.rng = np.random.default_rng(21)
+        for i in range(2):
+            assert rel_l2(res[i], oracle_mod.field_c(g, obj.get())) <= fp64_tol(g), (backend.__name__, i)
+    gnf = oracle.Geometry(24, 24, 75.0, 1e5, obj.k, nf=True, scale=True)
+    obj.rng = np.random.default_rng(22)
+    res = sim.cpu.simulate(2, obj, 24, 75.0, 1e5, settings="nf scale single")  # "single" is ignored: FP64
+    obj.rng = np.random.default_rng(22)
+    for i in range(2):
+        assert rel_l2(res[i], oracle_mod.field_c(gnf, obj.get())) <= fp64_tol(gnf)
+    gen = sim.cpu.simulate_gen(obj, (24, 20), 75.0, 1e5, maximg=3)
+    imgs = [next(gen) for _ in range(3)]
+    assert imgs[0] is imgs[1] is imgs[2] and imgs[0].dtype == np.complex128 and imgs[0].shape == (24, 20)
+    with pytest.raises(StopIteration):
+        next(gen)
+
+
+def test_handle_cache_eviction_keeps_live_handles():
+    """Nine geometries in use at once (the cache holds eight): a running simulate_gen and a
+    pending backward() keep working after their cached handles were evicted."""
+    import torch
+
+    from idi_amd.simulation import autograd, hip, simobj
+
+    hip._HANDLES.clear()
+    obj = simobj.sphere(8048.0, 200, 0.03)
+    obj.rng = np.random.default_rng(31)
+    gen = hip.simulate_gen(obj, 12, 75.0, 1e5, K_UM, settings="single nf", maximg=3)
+    first = next(gen)
+    pos = torch.tensor(obj.get()[:, :3], device="cuda", requires_grad=True)
+    ph = torch.zeros(200, dtype=torch.float64, device="cuda", requires_grad=True)
+    I = autograd.intensity(pos, ph, Ndet=10, pixelsize=80.0, detz=1e5, k=K_UM)
+    for j in range(9):  # nine more geometries evict both handles from the cache
+        hip.simulate(1, obj, 8 + j, 70.0, 1e5, K_UM, settings="single nf", verbose=False)
+    assert len(hip._HANDLES) == 8
+    rest = list(gen)
+    assert len(rest) == 2 and all(np.all(np.isfinite(r)) for r in [first] + rest)
+    I.sum().backward()
+    assert torch.isfinite(pos.grad).all() and torch.isfinite(ph.grad).all()

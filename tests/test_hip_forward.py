@@ -229,6 +229,59 @@ def test_extent_measured_on_device(lib):
     assert ext.value == pytest.approx(np.linalg.norm(at[:, :3], axis=1).max(), rel=1e-15)
 
 
+def test_extent_underestimate_is_reported(lib):
+    """An extent of half max|r_j| passed to the FP32-split kernels: the record stage flags it,
+    idisim_get_plan reports it once the kernel has finished, and the handle's next forward,
+    batched or adjoint call fails with IDISIM_E_PRECISION (then the flag is clear again).  A
+    correct extent, and the measured one (extent <= 0), raise nothing."""
+    import torch
+
+    from oracle import Geometry
+
+    rng = np.random.default_rng(9)
+    at = atoms_ball(rng, 3000, 0.01)
+    rmax = float(np.linalg.norm(at[:, :3], axis=1).max())
+    g = Geometry(64, 64, 75.0, 2e5, K_UM, nf=True)
+    h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+    d = torch.from_numpy(at).cuda()
+    for ext in (rmax, 0.0, 2 * rmax):
+        h.forward(d, extent=ext)
+        torch.cuda.synchronize()
+        assert h.plan()["extent_exceeded"] == 0, ext
+    good = h.forward(d, extent=rmax).cpu().numpy()
+
+    def next_call_fails(call):
+        torch.cuda.synchronize()
+        assert h.plan()["extent_exceeded"] == 1
+        with pytest.raises(lib.IdisimError) as ei:
+            call()
+        assert ei.value.code == lib.E_PRECISION and "extent" in str(ei.value)
+        assert h.plan()["extent_exceeded"] == 1  # the call that found the flag reports it
+        out = h.forward(d, extent=rmax)  # the flag was cleared: the next call runs
+        torch.cuda.synchronize()
+        assert h.plan()["extent_exceeded"] == 0
+        np.testing.assert_array_equal(out.cpu().numpy(), good)
+
+    # single shot, with a forced pair tile and the planner's choice
+    h.forward(d, extent=0.5 * rmax)
+    next_call_fails(lambda: h.forward(d, extent=rmax))
+    # batched: two shots, the flag found by the next batched call
+    offs = np.array([0, 1000, 3000])
+    h.forward_batched(d, offs, extent=0.5 * rmax)
+    next_call_fails(lambda: h.forward_batched(d, offs, extent=rmax))
+    # adjoint
+    f = h.forward(d, extent=rmax)
+    dl = torch.ones(g.n0, g.n1, dtype=torch.float32, device="cuda")
+    h.adjoint(d, f, dl, extent=0.5 * rmax)
+    next_call_fails(lambda: h.adjoint(d, f, dl, extent=rmax))
+    # an FP64 handle does not depend on the extent and checks nothing
+    h64 = make_handle(lib, g, lib.PREC_FP64)
+    h64.forward(d, extent=0.5 * rmax)
+    h64.forward(d, extent=0.5 * rmax)
+    torch.cuda.synchronize()
+    assert h64.plan()["extent_exceeded"] == 0
+
+
 def test_fallback_and_strict_precision(lib, oracle_mod):
     """Extreme near field (10 um sphere at 1 mm, the geometry of the reference's shape test)."""
     from oracle import Geometry
