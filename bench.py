@@ -42,12 +42,28 @@ FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector (published)
 OPS_PER_PAIR_NF = 23       # near field, unit amplitude: 3 sub, 3 mul + 2 add, sqrt, 4 (phase), cos, sin, 8 (Kahan)
 OPS_PER_PAIR_NF_AMP = 25
 OPS_PER_PAIR_ADJ = 33
-# VALU instructions (of them transcendental) issued per pair by the tile variants of the single-image
-# forward, from rocprofv3 SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 on the 512x512 image
-ISSUED_PER_PAIR = {2: {"valu": 7.28, "trans": 2.00}, 5: {"valu": 6.03, "trans": 1.50}, 9: {"valu": 6.21, "trans": 0.52},
-                   13: {"valu": 5.75, "trans": 0.52}}
-# the same for the persistent batched kernel on the 256-shot workload
-ISSUED_PER_PAIR_BATCHED = {8: {"valu": 7.16, "trans": 1.25}, 12: {"valu": 8.95, "trans": 0.52}}
+# The published FP32 peak counts an FMA as two flops: 157.3 TFLOP/s is 7.865e13 FMA issue slots per
+# second over the chip's lanes.  v_sin_f32 / v_cos_f32 issue at a quarter of that rate.
+PEAK_SLOTS_PER_S = FP32_PEAK_TFLOPS * 1e12 / 2
+TRANS_SLOTS = 4
+MIX_FILE = os.path.join(ROOT, "profiles", "valu_mix.json")
+
+
+def load_mix(family, tile, amp=False):
+    """VALU and transcendental instructions per pair of an FP32-split kernel, from the rocprofv3
+    counter summaries (profiles/valu_mix.json); None when no summary covers this kernel."""
+    try:
+        with open(MIX_FILE) as f:
+            entries = json.load(f)["entries"]
+    except (OSError, KeyError, ValueError):
+        return None
+    found = [e for e in entries if e["family"] == family and e["tile"] == tile]
+    found.sort(key=lambda e: e.get("amp", False) != bool(amp))  # the entry with the same amplitude mode first
+    if not found:
+        return None
+    e = found[0]
+    return {"valu": e["valu_per_pair"], "trans": e["trans_per_pair"], "kernel": e["kernel"], "source": e["source"],
+            "amp": e.get("amp", False)}
 
 
 def issue_fields(lib, mix, pairs_per_s):
@@ -60,6 +76,29 @@ def issue_fields(lib, mix, pairs_per_s):
         issue.update({"valu_per_pair": mix["valu"], "trans_per_pair": mix["trans"],
                       "ceiling_pairs_per_s": ceiling, "frac": pairs_per_s / ceiling})
     return issue
+
+
+def roofline_valu(lib, mix, pairs_per_s, ops_per_pair, traffic=None, kernel_ms=None):
+    """Roofline fields of a VALU-bound kernel.
+
+    achieved / frac: the FMA issue slots the kernel's instruction stream occupies per second, from its
+    measured mix (each VALU instruction one slot per lane, a transcendental TRANS_SLOTS), at two flops per
+    slot as the published peak counts them; it cannot exceed 1.  algorithmic: the reference formula's
+    ops per pair at the achieved pair rate (the mirror-pair tiles share work across pixels, so this one
+    can exceed the peak).  issue: the same mix against the v_fma_f32 / v_sin_f32 rates measured on this
+    device in the same run."""
+    r = {"bound": "valu", "achieved": None, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": None,
+         "traffic": traffic, "kernel_ms": kernel_ms}
+    if mix is not None:
+        slots = mix["valu"] - mix["trans"] + TRANS_SLOTS * mix["trans"]
+        achieved = 2.0 * slots * pairs_per_s / 1e12
+        r.update({"achieved": achieved, "frac": achieved / FP32_PEAK_TFLOPS, "slots_per_pair": slots,
+                  "valu_per_pair": mix["valu"], "trans_per_pair": mix["trans"], "mix_kernel": mix["kernel"],
+                  "mix_source": mix["source"]})
+    alg = pairs_per_s * ops_per_pair / 1e12
+    r["algorithmic"] = {"ops_per_pair": ops_per_pair, "tflops": alg, "frac": alg / FP32_PEAK_TFLOPS}
+    r["issue"] = issue_fields(lib, mix, pairs_per_s)
+    return r
 
 
 def log(*a):
@@ -200,11 +239,9 @@ def bench_single(args):
                "sample": f"{len(rows)} evenly spaced rows x {n1} px x {len(atoms)} emitters "
                          f"({len(rows) * n1 * len(atoms):.3e} pairs, {dt:.1f} s) of the same image; "
                          f"FP64 C/OpenMP restatement of cpu.py:30-107, {cpu_model()}"}
-    achieved = pairs * OPS_PER_PAIR_NF / (split_ms * 1e-3) / 1e12
-    # issue ceiling of the instruction mix the planned tile issues, from the v_fma_f32 and
-    # v_sin_f32 / v_cos_f32 rates measured on this device
-    issue = issue_fields(_lib, ISSUED_PER_PAIR.get(plan["tile"]), pairs / (split_ms * 1e-3))
-    log("issue rates", issue)
+    rate = pairs / (split_ms * 1e-3)
+    roof = roofline_valu(_lib, load_mix("single", plan["tile"]), rate, OPS_PER_PAIR_NF, kernel_ms=split_ms)
+    log("roofline", roof)
     traffic = args.traffic
     if traffic is None:
         # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE for this plan
@@ -242,14 +279,9 @@ def bench_single(args):
         "rel_l2_intensity": {"fp32split_vs_fp64_gpu_full": rel_vs_fp64,
                              "fp32split_vs_oracle_rows": rel_vs_oracle,
                              "fp64_gpu_vs_oracle_rows": rel64_vs_oracle},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "ops_per_pair": OPS_PER_PAIR_NF, "kernel_ms": split_ms,
-                     # ops_per_pair counts the reference's per-pair formula; the mirror-pair tiles share
-                     # sincos and phase work across pixels, so they issue fewer instructions per pair than
-                     # that count and frac can exceed 1
-                     "issue": issue,
-                     "fp64_frac": pairs * OPS_PER_PAIR_NF / (float(np.mean(ms64)) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
+        "roofline": dict(roof, traffic=traffic,
+                         fp64_algorithmic_frac=pairs * OPS_PER_PAIR_NF / (float(np.mean(ms64)) * 1e-3) / 1e12 /
+                         FP64_PEAK_TFLOPS),
         "cpu_baseline": cpu,
     }
     return result
@@ -299,6 +331,24 @@ def bench_batched(args):
     f64 = {"value": pairs * f64_steps / wall64, "ms_per_step": wall64 * 1e3 / f64_steps, "plan": hf.plan()}
     print(f"fp64 batched: {f64['ms_per_step']:.1f} ms/step; relL2(I) fp32-split vs fp64 over 256 shots {rel:.3e}",
           file=sys.stderr)
+    # the FP64 oracle on rows of the smallest, a middle and the largest shot of the last step
+    rel_oracle = None
+    if not args.no_cpu:
+        import oracle
+
+        geom = oracle.Geometry(n0, n1, 100.0, 1e5, K, nf=True)
+        order = np.argsort(sizes)
+        shots = [int(order[0]), int(order[nshots // 2]), int(order[-1])]
+        rows = [0, 97, 128, 255]
+        host = d.cpu().numpy()
+        got, ref = [], []
+        for s_ in shots:
+            at = host[offsets[s_]:offsets[s_ + 1]]
+            ref.append(np.abs(oracle.field_c(geom, at, rows=rows, nthreads=host_threads())) ** 2)
+            got.append(out[s_].cpu().numpy()[rows])
+        rel_oracle = {"shots": shots, "emitters": [int(sizes[s_]) for s_ in shots], "rows": rows,
+                      "rel_l2": rel_l2(np.stack(got), np.stack(ref))}
+        log("batched vs oracle rows", rel_oracle)
     return {"metric": METRIC + " [batched 256 shots]", "value": pairs * args.steps / wall, "unit": "pair-evals/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -307,13 +357,9 @@ def bench_batched(args):
                                    "near field, intensity out", "pairs_per_step": pairs,
                        "plan": h.plan()},
             "fp64": f64, "speedup_vs_fp64": (pairs * args.steps / wall) / f64["value"],
-            "rel_l2_intensity": {"fp32split_vs_fp64_gpu_all_shots": rel},
-            "roofline": {"bound": "valu", "achieved": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12,
-                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": pairs * OPS_PER_PAIR_NF / (np.mean(ms) * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
-                         "traffic": None,
-                         "issue": issue_fields(_lib, ISSUED_PER_PAIR_BATCHED.get(h.plan()["tile"]),
-                                               pairs / (np.mean(ms) * 1e-3))}}
+            "rel_l2_intensity": {"fp32split_vs_fp64_gpu_all_shots": rel, "fp32split_vs_oracle_rows": rel_oracle},
+            "roofline": roofline_valu(_lib, load_mix("batched", h.plan()["tile"]), pairs / (np.mean(ms) * 1e-3),
+                                      OPS_PER_PAIR_NF, kernel_ms=float(np.mean(ms)))}
 
 
 def three_balls(rng, n_total=1_000_000):
@@ -358,7 +404,22 @@ def bench_large(args):
     timer = StepTimer()
     timer.run(step, args.warmup)
     wall, ms = timer.run(step, args.steps)
-    ach = pairs * OPS_PER_PAIR_NF_AMP / (np.mean(ms) * 1e-3) / 1e12
+    # the FP64 oracle on two rows of the first and the last shot of the last step, at the full emitter count
+    rel_oracle = None
+    if not args.no_cpu:
+        import oracle
+
+        geom = oracle.Geometry(n0, n1, 50.0, 5e5, K, nf=True)
+        rows = [300, 512]
+        got, ref = [], []
+        ahost = amp.cpu().numpy()
+        for s_ in (0, nshots - 1):
+            at = d[s_ * n:(s_ + 1) * n].cpu().numpy()
+            ref.append(np.abs(oracle.field_c(geom, at, ahost[s_ * n:(s_ + 1) * n], rows=rows,
+                                             nthreads=host_threads())) ** 2)
+            got.append(out[s_].cpu().numpy()[rows])
+        rel_oracle = {"shots": [0, nshots - 1], "rows": rows, "rel_l2": rel_l2(np.stack(got), np.stack(ref))}
+        log("large vs oracle rows", rel_oracle)
     return {"metric": METRIC + " [1e6 emitters, 16 shots, amplitudes]", "value": pairs * args.steps / wall,
             "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -366,8 +427,9 @@ def bench_large(args):
             "config": {"workload": "1e6 emitters in three balls (r=15/20/25 nm, counts %s), amplitudes U[0.5,1], "
                                    "fresh phases, 16 shots, 1024x1024 px of 50 um at z=0.5 m, near field"
                                    % counts.tolist(), "pairs_per_step": pairs, "plan": h.plan()},
-            "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "ops_per_pair": OPS_PER_PAIR_NF_AMP}}
+            "rel_l2_intensity": {"fp32split_vs_oracle_rows": rel_oracle},
+            "roofline": roofline_valu(_lib, load_mix("batched", h.plan()["tile"], amp=True),
+                                      pairs / (np.mean(ms) * 1e-3), OPS_PER_PAIR_NF_AMP, kernel_ms=float(np.mean(ms)))}
 
 
 def bench_adjoint(args):
@@ -403,13 +465,43 @@ def bench_adjoint(args):
     timer = StepTimer()
     timer.run(step, args.warmup)
     wall, ms = timer.run(step, args.steps)
+    # the gradients of one step against long-double central differences of the loss (h = 1e-4 nm for
+    # positions, 1e-4 rad for phases) on 24 position and 8 phase coordinates, and against the FP64
+    # analytic gradient of the oracle over all 4N coordinates
+    check = None
+    if not args.no_cpu:
+        import oracle
+
+        geom = oracle.Geometry(n0, n1, 75.0, 1e5, K, nf=True)
+        h.forward(d, None, field, _lib.OUT_FIELD, ext)
+        I = torch.view_as_real(field).square().sum(-1)
+        gpos, gph, _ = h.adjoint(d, field, torch.sub(I, target32).mul_(2.0), None, ext, want_amp=False)
+        gpos, gph = gpos.cpu().numpy().astype(np.float64), gph.cpu().numpy().astype(np.float64)
+        T = target.cpu().numpy()
+        crng = np.random.default_rng(4001)
+        sel = [(int(j), int(c)) for j, c in zip(crng.choice(n, 32, replace=False), crng.integers(0, 3, 32))]
+        sel = sel[:24] + [(j, 3) for j, _ in sel[24:]]
+        nt = host_threads()
+        t0 = time.perf_counter()
+        fd = oracle.loss_central_diff(geom, at, T, sel, [1e-7 if c < 3 else 1e-4 for _, c in sel], nthreads=nt)
+        fd_s = time.perf_counter() - t0
+        gpu = np.array([gpos[j, c] if c < 3 else gph[j] for j, c in sel])
+        rms = np.array([np.sqrt(np.mean(gpos**2)) if c < 3 else np.sqrt(np.mean(gph**2)) for _, c in sel])
+        fref = oracle.field_c(geom, at, nthreads=nt)
+        rpos, rph, _ = oracle.adjoint_c(geom, at, 2 * (np.abs(fref) ** 2 - T), field=fref, nthreads=nt)
+        check = {"fd_coordinates": len(sel), "fd_seconds": fd_s,
+                 "max_rel_err_vs_fd": float(np.max(np.abs(gpu - fd) / np.abs(fd))),
+                 "max_err_vs_fd_over_abs_plus_rms": float(np.max(np.abs(gpu - fd) / (np.abs(fd) + rms))),
+                 "rel_l2_vs_fp64_analytic": {"pos": rel_l2(gpos, rpos), "phase": rel_l2(gph, rph)}}
+        log("adjoint check", check)
     return {"metric": METRIC + " [forward + adjoint]", "value": pairs * args.steps / wall, "unit": "pair-evals/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
             "config": {"workload": "4096 emitters in a 20 nm ball perturbed by N(0, 0.05 nm), loss "
                                    "sum (I - I_target)^2, 256x256 px of 75 um at z=0.1 m, near field; "
                                    "step = forward + adjoint (pairs counted twice)", "pairs_per_step": pairs,
-                       "plan": h.plan()}}
+                       "plan": h.plan()},
+            "gradient_check": check}
 
 
 def bench_c1(args):
@@ -728,13 +820,32 @@ def bench_generate(args):
             "gauss": rates["gauss"], "choice": rates["choice"], "roofline": None, "cpu_baseline": cpu}
 
 
+def bench_issue(args):
+    """The issue-rate probes alone (idisim_issue_rates: eight independent v_fma_f32 or v_sin_f32 /
+    v_cos_f32 chains per lane, 16 workgroups per CU), --steps times: the rates the FP32-split kernels'
+    issue ceilings are computed from, in a run short enough to profile with counters."""
+    from idi_amd import _lib
+
+    rates = [_lib.issue_rates() for _ in range(max(1, args.steps))]
+    fma = max(r[0] for r in rates)
+    trans = max(r[1] for r in rates)
+    return {"metric": "VALU lane operations/s [issue-rate probes]", "value": fma, "unit": "v_fma_f32 lane ops/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "issue-rate probes: 16 workgroups of 256 lanes per CU, eight independent chains "
+                                   "per lane"},
+            "trans_per_s": trans, "fma_per_s_runs": [r[0] for r in rates], "trans_per_s_runs": [r[1] for r in rates],
+            "published_fma_slots_per_s": PEAK_SLOTS_PER_S, "fma_frac_of_published": fma / PEAK_SLOTS_PER_S,
+            "roofline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="single", choices=["single", "c1", "batched", "large", "adjoint", "timedep", "e2e", "photons",
-                                                         "correlate", "generate"])
+                                                         "correlate", "generate", "issue"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile", type=int, default=None,
                     help="single-image workload: run this FP32-split tile variant instead of the planner's choice")
@@ -755,7 +866,8 @@ def main():
         sys.exit(1)
     res = {"single": bench_single, "batched": bench_batched, "large": bench_large,
            "adjoint": bench_adjoint, "timedep": bench_timedep, "c1": bench_c1, "e2e": bench_e2e,
-           "photons": bench_photons, "correlate": bench_correlate, "generate": bench_generate}[args.workload](args)
+           "photons": bench_photons, "correlate": bench_correlate, "generate": bench_generate,
+           "issue": bench_issue}[args.workload](args)
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.rand(n, device="cuda")

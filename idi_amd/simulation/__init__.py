@@ -29,8 +29,8 @@ __all__ = ["hip", "cpu", "cuda", "simobj", "auto", "autotime", "autograd", "comm
 
 
 def __getattr__(name):
-    if name == "autograd":
-        from . import autograd
+    if name == "autograd":  # imported on first use: it imports torch
+        import importlib
 
-        return autograd
+        return importlib.import_module(".autograd", __name__)
     raise AttributeError(name)

@@ -13,14 +13,19 @@ Contents
     ``idi/simulation/cpu.py:30-107`` (small sizes).
   * :func:`field_c` / :func:`adjoint_c` -- ctypes wrappers of the OpenMP C
     restatement in ``wavefield.c`` (also the timed CPU baseline).
+  * :func:`loss_central_diff` -- long-double central differences of the
+    squared-intensity loss, the independent check of the gradients.
 
 How each function is pinned (checked by ``tests/test_oracle.py``):
   * field_numpy, field_c: mpmath 50-digit golden vectors (tests/golden), the
     closed forms single emitter, two emitters, emitters at the origin, Laue
     lattice sum; translation, phase-shift and Friedel invariants; numpy vs C
     brute-force agreement.
-  * adjoint_c: FP64 central differences of field_c, and the closed-form
+  * adjoint_c: long-double central differences of the loss, and the closed-form
     gradient of a single emitter.
+  * loss_central_diff: an independent pure-NumPy long-double evaluation of the
+    loss at r +/- h on tiny cases, and the closed-form phase gradient of two
+    emitters.
   * rotation: orthogonality, round trip through angles, known matrices.
 """
 from __future__ import annotations
@@ -182,6 +187,11 @@ def _load():
             ctypes.POINTER(_CGeom), dp, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, dp, ctypes.c_int,
         ]
         lib.oracle_time_intensity.restype = None
+        lib.oracle_loss_central_diff.argtypes = [
+            ctypes.POINTER(_CGeom), dp, dp, ctypes.c_int64, dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, dp,
+            dp, ctypes.c_int,
+        ]
+        lib.oracle_loss_central_diff.restype = None
         _lib = lib
     return _lib
 
@@ -239,6 +249,27 @@ def adjoint_c(geom: Geometry, atoms, dLdI, field=None, amp=None, nthreads=0):
     lib.oracle_adjoint(ctypes.byref(g), _dptr(atoms), _dptr(amp), n, _dptr(f2), _dptr(dLdI),
                        _dptr(gpos), _dptr(gph), _dptr(gamp), int(nthreads))
     return gpos, gph, gamp
+
+
+def loss_central_diff(geom: Geometry, atoms, target, coords, h, amp=None, nthreads=0):
+    """Central differences (L(+h) - L(-h)) / 2h of L = sum_p (|E_p|^2 - T_p)^2 in 80-bit long
+    double (oracle/wavefield.c: the base field once, then only the moved emitter's term per
+    coordinate).  coords: (j, c) pairs, c = 0, 1, 2 for x, y, z and 3 for the phase; h: one
+    step or one per coordinate (length unit / radians).  Returns float64 (len(coords),)."""
+    lib = _load()
+    atoms = np.ascontiguousarray(atoms, dtype=np.float64)
+    amp = None if amp is None else np.ascontiguousarray(amp, dtype=np.float64)
+    target = np.ascontiguousarray(target, dtype=np.float64).reshape(geom.n0, geom.n1)
+    cs = np.ascontiguousarray(coords, dtype=np.int64).reshape(-1, 2)
+    if np.any(cs[:, 0] < 0) or np.any(cs[:, 0] >= len(atoms)) or np.any(cs[:, 1] < 0) or np.any(cs[:, 1] > 3):
+        raise ValueError("coords must be (emitter, component) pairs with component 0..3")
+    hs = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (len(cs),)))
+    out = np.empty(len(cs))
+    g = _cgeom(geom)
+    lib.oracle_loss_central_diff(ctypes.byref(g), _dptr(atoms), _dptr(amp), len(atoms), _dptr(target),
+                                 cs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(cs), _dptr(hs),
+                                 _dptr(out), int(nthreads))
+    return out
 
 
 def time_c(geom: Geometry, atoms, times, c, tau, nthreads=0):

@@ -213,4 +213,139 @@ void oracle_adjoint(const oracle_geom *g, const double *atoms, const double *amp
     free(pp); free(ff); free(ww);
 }
 
+/*
+ * Central differences of the loss L = sum_p (I_p - T_p)^2 (I_p = |E_p|^2) for selected
+ * coordinates, in 80-bit long double: the check of the analytic gradient above that shares
+ * none of its derivative formulas.
+ *
+ * E_p is summed once.  Moving coordinate c of emitter j by s h (s = +1, -1) changes only that
+ * emitter's term t = a m e^{i theta}:  E_p(s) = E_p + Delta_s,
+ *   Delta_s = a e^{i theta} [ (m_s - m) e^{i d_s} + m (e^{i d_s} - 1) ],  d_s = theta_s - theta,
+ * with e^{i d} - 1 = -2 sin^2(d / 2) + i sin d, and, in the near field, D_s - D =
+ * (D_s^2 - D^2) / (D_s + D), D_s^2 - D^2 = -2 s h (r_p - r_j)_c + h^2, so that no difference
+ * of two large numbers is formed.  Then
+ *   L(+h) - L(-h) = sum_p (I+ - I-) (I+ + I- - 2 T_p),   I+- = |E_p + Delta_+-|^2,
+ * where I+ - I- (a few 1e-4 of I at h = 1e-4 nm) keeps ~15 digits in long double.
+ *   coords: ncoord pairs (j, c); c = 0, 1, 2: position x, y, z; 3: phase
+ *   hstep:  ncoord steps (length unit for positions, radians for phases)
+ *   out:    ncoord quotients (L(+h) - L(-h)) / (2 h)
+ */
+typedef long double ld;
+
+static void pixel_position_ld(const oracle_geom *g, int64_t p, ld *r)
+{
+    double x, y, z;
+    pixel_position(g, p / g->n1, p % g->n1, &x, &y, &z);
+    r[0] = x; r[1] = y; r[2] = z;
+}
+
+/* theta and m of one term in long double (cpu.py:44-54, 59-65, sim.cu:88) */
+static void term_ld(const oracle_geom *g, const ld *rp, const ld *a4, ld *theta, ld *m, ld *dist)
+{
+    const ld k = (ld)g->k;
+    if (g->nf) {
+        const ld dx = rp[0] - a4[0], dy = rp[1] - a4[1], dz = rp[2] - a4[2];
+        const ld D = sqrtl(dx * dx + dy * dy + dz * dz);
+        *theta = (D - rp[2] + a4[2]) * k + a4[3];
+        *m = g->scale ? 1.0L / D : 1.0L;
+        *dist = D;
+    } else {
+        const ld R = sqrtl(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]);
+        const ld qx = k * rp[0] / R, qy = k * rp[1] / R, qz = k * (rp[2] / R - 1.0L);
+        ld th = -(a4[0] * qx + a4[1] * qy + a4[2] * qz) + a4[3];
+        if (g->secondorder) th += k / (2.0L * R) * (a4[0] * a4[0] + a4[1] * a4[1]);
+        *theta = th;
+        *m = g->scale ? 1.0L / R : 1.0L;
+        *dist = R;
+    }
+}
+
+/* d = theta_s - theta and m_s - m for coordinate c of the emitter moved by step hs */
+static void term_shift_ld(const oracle_geom *g, const ld *rp, const ld *a4, int c, ld hs, ld D,
+                          ld *d, ld *dm)
+{
+    const ld k = (ld)g->k;
+    *dm = 0.0L;
+    if (c == 3) {
+        *d = hs;
+        return;
+    }
+    if (g->nf) {
+        const ld diff = rp[c] - a4[c];              /* (r_p - r_j)_c */
+        const ld dD2 = -2.0L * hs * diff + hs * hs;  /* D_s^2 - D^2 */
+        ld D2 = 0.0L;
+        for (int i = 0; i < 3; ++i) {
+            const ld v = rp[i] - a4[i] - (i == c ? hs : 0.0L);
+            D2 += v * v;
+        }
+        const ld Ds = sqrtl(D2);
+        const ld dD = dD2 / (Ds + D);
+        *d = k * (dD + (c == 2 ? hs : 0.0L));
+        if (g->scale) *dm = -dD / (D * Ds);
+    } else {
+        const ld R = D;
+        const ld q[3] = {k * rp[0] / R, k * rp[1] / R, k * (rp[2] / R - 1.0L)};
+        ld dd = -q[c] * hs;
+        if (g->secondorder && c < 2) dd += k / (2.0L * R) * (2.0L * a4[c] * hs + hs * hs);
+        *d = dd;
+    }
+}
+
+void oracle_loss_central_diff(const oracle_geom *g, const double *atoms, const double *amp, int64_t n,
+                              const double *target, const int64_t *coords, int64_t ncoord, const double *hstep,
+                              double *out, int nthreads)
+{
+    const int64_t npix = g->n0 * g->n1;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    ld *E = (ld *)malloc(sizeof(ld) * 2 * npix);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < npix; ++p) {
+        ld rp[3];
+        pixel_position_ld(g, p, rp);
+        ld er = 0.0L, ei = 0.0L;
+        for (int64_t j = 0; j < n; ++j) {
+            const ld a4[4] = {atoms[4 * j], atoms[4 * j + 1], atoms[4 * j + 2], atoms[4 * j + 3]};
+            const ld a = amp ? (ld)amp[j] : 1.0L;
+            ld th, m, D;
+            term_ld(g, rp, a4, &th, &m, &D);
+            er += a * m * cosl(th);
+            ei += a * m * sinl(th);
+        }
+        E[2 * p] = er;
+        E[2 * p + 1] = ei;
+    }
+    for (int64_t i = 0; i < ncoord; ++i) {
+        const int64_t j = coords[2 * i];
+        const int c = (int)coords[2 * i + 1];
+        const ld h = (ld)hstep[i];
+        const ld a4[4] = {atoms[4 * j], atoms[4 * j + 1], atoms[4 * j + 2], atoms[4 * j + 3]};
+        const ld a = amp ? (ld)amp[j] : 1.0L;
+        ld sum = 0.0L;
+#pragma omp parallel for schedule(static) reduction(+ : sum)
+        for (int64_t p = 0; p < npix; ++p) {
+            ld rp[3];
+            pixel_position_ld(g, p, rp);
+            ld th, m, D;
+            term_ld(g, rp, a4, &th, &m, &D);
+            const ld ct = cosl(th), st = sinl(th);
+            ld I[2];
+            for (int s = 0; s < 2; ++s) {
+                ld d, dm;
+                term_shift_ld(g, rp, a4, c, s == 0 ? h : -h, D, &d, &dm);
+                const ld sh = sinl(0.5L * d);
+                const ld e_re = -2.0L * sh * sh, e_im = sinl(d);  /* e^{i d} - 1 */
+                /* (m_s - m) e^{i d} + m (e^{i d} - 1) */
+                const ld u_re = dm * (1.0L + e_re) + m * e_re;
+                const ld u_im = dm * e_im + m * e_im;
+                const ld dr = a * (ct * u_re - st * u_im), di = a * (ct * u_im + st * u_re);
+                const ld fr = E[2 * p] + dr, fi = E[2 * p + 1] + di;
+                I[s] = fr * fr + fi * fi;
+            }
+            sum += (I[0] - I[1]) * (I[0] + I[1] - 2.0L * (ld)target[p]);
+        }
+        out[i] = (double)(sum / (2.0L * h));
+    }
+    free(E);
+}
+
 int oracle_abi_version(void) { return 1; }

@@ -117,3 +117,14 @@ def test_inputs_on_another_device_are_rejected_before_any_launch():
         h.time_forward(emit, torch.zeros(5, dtype=torch.float64), 1.0, 1.0)
     with pytest.raises(ValueError, match="(?i)cuda"):
         h._check_out(torch.zeros((4, 4), dtype=torch.complex64), _lib.OUT_FIELD, 1)
+
+
+def test_simulation_namespace():
+    """The names idi.simulation offers (simulation/__init__.py:4-31) and the ones the reference
+    scripts call: auto, cuda and cpu, and the lazily imported autograd module."""
+    from idi_amd import simulation as sim
+    from idi_amd.simulation import autograd
+
+    assert sim.auto is sim.hip and sim.cuda is sim.hip
+    assert sim.cpu.simulate.__module__ == "idi_amd.simulation.cpu"
+    assert sim.autograd is autograd and callable(autograd.intensity)

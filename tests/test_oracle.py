@@ -289,16 +289,28 @@ def test_adjoint_vs_central_differences(oracle_mod, mode):
     dLdI = 2 * (np.abs(field) ** 2 - target)
     gpos, gph, gamp = oracle_mod.adjoint_c(g, at, dLdI, field=field, amp=a)
     # h = 1e-4 nm (k h |grad| ~ 4e-3 rad): truncation ~1e-6 relative
+    coords = [(j, c) for j in range(6) for c in range(4)]
+    floor = np.abs(gpos).max() * 1e-2 + 1e-300
     for h, rtol in ((1e-7, 2e-5), (1e-6, 1e-4)):
-        for j in range(6):
-            for c in range(4):
-                p, m = at.copy(), at.copy()
-                hh = h if c < 3 else 1e-4
-                p[j, c] += hh
-                m[j, c] -= hh
-                fd = (loss(p, a) - loss(m, a)) / (2 * hh)
-                an = gpos[j, c] if c < 3 else gph[j]
-                assert abs(fd - an) <= rtol * (abs(an) + np.abs(gpos).max() * 1e-2 + 1e-300), (h, j, c, fd, an)
+        steps = [h if c < 3 else 1e-4 for _, c in coords]
+        # the C long-double differences (the base field once, the moved term's change in closed form)
+        cfd = oracle_mod.loss_central_diff(g, at, target, coords, steps, amp=a)
+        for i, ((j, c), hh) in enumerate(zip(coords, steps)):
+            p, m = at.copy(), at.copy()
+            p[j, c] += hh
+            m[j, c] -= hh
+            fd = (loss(p, a) - loss(m, a)) / (2 * hh)
+            an = gpos[j, c] if c < 3 else gph[j]
+            assert abs(fd - an) <= rtol * (abs(an) + floor), (h, j, c, fd, an)
+            assert abs(cfd[i] - an) <= rtol * (abs(an) + floor), (h, j, c, cfd[i], an)
+            if h == 1e-6:  # where the direct NumPy quotient's own rounding noise is ~1e-6 of it
+                assert abs(cfd[i] - fd) <= 1e-5 * (abs(fd) + floor), (h, j, c, cfd[i], fd)
+    # the C quotient converges as h^2 (no rounding noise down to h = 1e-5 nm)
+    pos = [(j, c) for j, c in coords if c < 3]
+    d7, d8 = (oracle_mod.loss_central_diff(g, at, target, pos, h, amp=a) for h in (1e-7, 1e-8))
+    d6 = oracle_mod.loss_central_diff(g, at, target, pos, 1e-6, amp=a)
+    assert np.all(np.abs(d7 - d8) <= 2e-2 * np.abs(d6 - d8) + 1e-9 * floor)
+    for j in range(6):
         ap, am = a.copy(), a.copy()
         ap[j] += 1e-6
         am[j] -= 1e-6
