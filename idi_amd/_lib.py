@@ -89,6 +89,7 @@ def load():
     lib.idisim_set_option.argtypes = [vp, ip, i64]
     lib.idisim_get_plan.argtypes = [vp, ctypes.POINTER(PlanT)]
     lib.idisim_forward.argtypes = [vp, vp, vp, i64, dbl, vp, ip, vp]
+    lib.idisim_forward_noalloc.argtypes = [vp, vp, vp, i64, dbl, vp, ip, vp]
     lib.idisim_forward_batched.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_int64), i64, dbl, vp, ip, vp]
     lib.idisim_adjoint.argtypes = [vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp, vp]
     lib.idisim_extent.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), vp]
@@ -106,6 +107,7 @@ def load():
     lib.idisim_generate_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
+                 "idisim_forward_noalloc",
                  "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
                  "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss",
                  "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward", "idisim_generate_choice",
@@ -381,6 +383,22 @@ class Handle:
         self._check_out(out, kind, 1)
         check(self._lib.idisim_forward(self._h, _ptr(emit), _ptr(amp), int(emit.shape[0]), float(extent), _ptr(out),
                                        int(kind), _stream()))
+        return out
+
+    def forward_noalloc(self, emit, out, amp=None, kind=OUT_FIELD, extent=0.0):
+        """idisim_forward_noalloc: as forward(), into a caller-given `out`, with the library's
+        static device scratch (no device allocation in the call; calls on one device must not
+        overlap)."""
+        import torch
+
+        if emit.dtype != torch.float64 or emit.dim() != 2 or emit.shape[1] != 4 or not emit.is_contiguous():
+            raise ValueError("emitters must be a contiguous (N, 4) float64 tensor")
+        if amp is not None and (amp.dtype != torch.float64 or amp.shape != (emit.shape[0],) or not amp.is_contiguous()):
+            raise ValueError("amplitudes must be a contiguous (N,) float64 tensor")
+        self._on_device(emitters=emit, amplitudes=amp)
+        self._check_out(out, kind, 1)
+        check(self._lib.idisim_forward_noalloc(self._h, _ptr(emit), _ptr(amp), int(emit.shape[0]), float(extent),
+                                               _ptr(out), int(kind), _stream()))
         return out
 
     def forward_batched(self, emit, offsets, amp=None, out=None, kind=OUT_FIELD, extent=0.0):

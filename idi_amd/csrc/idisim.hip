@@ -21,6 +21,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -101,6 +102,7 @@ struct idisim_handle {
     unsigned int* h_viol = nullptr;   // host view
     unsigned int* d_viol = nullptr;   // device view of the same word
     int viol_seen = 0;                // the flag was found set (reported by idisim_get_plan)
+    void* noalloc = nullptr;          // this device's address of the static no-alloc scratch
     // batched work items
     WorkItem* d_items = nullptr;
     size_t items_cap = 0;
@@ -365,16 +367,22 @@ int ensure_scratch(idisim_handle* h, size_t bytes)
     return IDISIM_OK;
 }
 
-int measure_extent(idisim_handle* h, const double* emit, int64_t n, hipStream_t st, double* rho)
+// max_j |r_j| of the call's emitters, reduced into the device word `word` (the handle's own
+// when NULL); synchronises the stream.
+int measure_extent(idisim_handle* h, const double* emit, int64_t n, hipStream_t st, double* rho,
+                   unsigned long long* word = nullptr)
 {
     if (n <= 0) {
         *rho = 0.0;
         return IDISIM_OK;
     }
-    if (!h->d_ext) IDISIM_HIP(hipMalloc(&h->d_ext, sizeof(unsigned long long)));
-    IDISIM_HIP(launch_extent(emit, n, h->d_ext, st));
+    if (word == nullptr) {
+        if (!h->d_ext) IDISIM_HIP(hipMalloc(&h->d_ext, sizeof(unsigned long long)));
+        word = h->d_ext;
+    }
+    IDISIM_HIP(launch_extent(emit, n, word, st));
     unsigned long long bits = 0;
-    IDISIM_HIP(hipMemcpyAsync(&bits, h->d_ext, sizeof(bits), hipMemcpyDeviceToHost, st));
+    IDISIM_HIP(hipMemcpyAsync(&bits, word, sizeof(bits), hipMemcpyDeviceToHost, st));
     IDISIM_HIP(hipStreamSynchronize(st));
     double r2;
     std::memcpy(&r2, &bits, sizeof(r2));
@@ -447,11 +455,22 @@ struct FwdCall {
     hipStream_t st;
     bool split_path;  // FP32-split kernels (false: FP64 kernels)
     bool out_f32;     // FP64 kernels writing a complex64 / float32 output (fallback)
+    // fixed device scratch of idisim_forward_noalloc (nullptr: the handle's own, grown on demand)
+    void* fixed = nullptr;
+    size_t fixed_cap = 0;
 };
 
-// Device scratch of `bytes` for one call: the handle's buffer, grown on demand.
-int call_scratch(idisim_handle* h, const FwdCall&, size_t bytes, void** p)
+// Device scratch of `bytes` for one call: the fixed buffer of a no-alloc call, which is never
+// grown, or the handle's buffer.
+int call_scratch(idisim_handle* h, const FwdCall& c, size_t bytes, void** p)
 {
+    if (c.fixed != nullptr) {
+        if (bytes > c.fixed_cap)
+            return fail(IDISIM_E_INVALID, "call needs " + std::to_string(bytes) + " bytes of scratch, the static buffer holds " +
+                                              std::to_string(c.fixed_cap));
+        *p = c.fixed;
+        return IDISIM_OK;
+    }
     int rc = ensure_scratch(h, bytes);
     if (rc) return rc;
     *p = h->scratch;
@@ -538,7 +557,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
         ch.split = h->force_split > 0 ? h->force_split : heuristic_split(h, c.n, ch.tile);
     } else if (it != h->tuned.end() && has_tile(tiles, it->second.tile)) {
         ch = it->second;
-    } else if (h->autotune) {
+    } else if (h->autotune && c.fixed == nullptr) {
         // time candidates on (at most) ~2e10 pairs of this call's work
         const size_t npix = (size_t)h->g.n0 * h->g.n1;
         int64_t nt = c.n;
@@ -589,6 +608,11 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     }
     const int64_t maxs = std::max<int64_t>(1, (c.n + kSplitGrain - 1) / kSplitGrain);
     if (ch.split > maxs) ch.split = (int)maxs;
+    if (c.fixed != nullptr && ch.split > 1) {
+        // a no-alloc call: as many splits as the static buffer holds partials for
+        const size_t per = (size_t)h->g.n0 * h->g.n1 * (c.split_path ? sizeof(float4) : sizeof(double4));
+        ch.split = (int)std::max<size_t>(1, std::min<size_t>((size_t)ch.split, c.fixed_cap / per));
+    }
     int rc = run_forward(h, c, ch.tile, ch.split);
     if (rc) return rc;
     if (c.split_path) h->plan.phase_bound = plan_bound(h, ch.tile, h->rho);  // the tile that ran
@@ -750,7 +774,34 @@ int check_extent_flag(idisim_handle* h, const char* who)
 
 }  // namespace
 
+// Static device scratch of idisim_forward_noalloc: the partial sums of the emitter splits
+// (float4 or double4 per pixel and split).  Each device holds its own copy; host code uses
+// only the address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[IDISIM_NOALLOC_SCRATCH_BYTES];
 
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), resolved once per device.
+int noalloc_scratch(int device, void** p)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(IDISIM_E_INVALID, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device] == nullptr) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            return fail(IDISIM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *p = g_scratch_ptr[device];
+    return IDISIM_OK;
+}
+
+}  // namespace
 
 // ============================== C ABI =======================================
 extern "C" {
@@ -861,6 +912,13 @@ int idisim_create(idisim_handle** out, const idisim_geometry* geom, int precisio
         (void)hipHostFree(h->h_viol);
         delete h;
         return fail(IDISIM_E_HIP, "idisim_create: hipHostGetDevicePointer failed");
+    }
+    // the static scratch of idisim_forward_noalloc, resolved here once per device
+    int rc = noalloc_scratch(device, &h->noalloc);
+    if (rc) {
+        (void)hipHostFree(h->h_viol);
+        delete h;
+        return rc;
     }
     *out = h;
     return IDISIM_OK;
@@ -1133,17 +1191,22 @@ static int choose_path(idisim_handle* h, double rho, std::vector<int>* tiles, bo
     return IDISIM_OK;
 }
 
+// idisim_forward (noalloc false) and idisim_forward_noalloc (true: the partial sums and the
+// extent word live in the static scratch, the plan is the tuned or heuristic one without a
+// timing pass, and the emitter split is capped by the scratch size).
 static int forward_call(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
-                        void* out, int out_kind, hipStream_t st)
+                        void* out, int out_kind, hipStream_t st, bool noalloc)
 {
+    const char* who = noalloc ? "idisim_forward_noalloc" : "idisim_forward";
     DeviceGuard guard(h->device);
-    int rc = check_extent_flag(h, "idisim_forward");
+    int rc = check_extent_flag(h, who);
     if (rc) return rc;
     const size_t npix = (size_t)h->g.n0 * h->g.n1;
     if (n == 0) return fill_zero(out, npix * out_elem_bytes(h->precision, out_kind), st);
     double rho = extent;
     if (!(rho > 0.0)) {
-        rc = measure_extent(h, emitters, n, st, &rho);
+        rc = measure_extent(h, emitters, n, st, &rho,
+                            noalloc ? static_cast<unsigned long long*>(h->noalloc) : nullptr);
         if (rc) return rc;
     }
     std::vector<int> tiles;
@@ -1152,6 +1215,10 @@ static int forward_call(idisim_handle* h, const double* emitters, const double* 
     if (rc) return rc;
     FwdCall c{emitters, amplitudes, n, out, out_kind, st, split_path,
               h->precision == IDISIM_PREC_FP32SPLIT && !split_path};
+    if (noalloc) {
+        c.fixed = h->noalloc;
+        c.fixed_cap = IDISIM_NOALLOC_SCRATCH_BYTES;
+    }
     return plan_and_run(h, c, tiles);
 }
 
@@ -1165,7 +1232,21 @@ int idisim_forward(idisim_handle* h, const double* emitters, const double* ampli
     if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
         return fail(IDISIM_E_INVALID, "idisim_forward: unknown output kind");
     if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward: extent must be finite");
-    return forward_call(h, emitters, amplitudes, n, extent, out, out_kind, static_cast<hipStream_t>(stream));
+    return forward_call(h, emitters, amplitudes, n, extent, out, out_kind, static_cast<hipStream_t>(stream), false);
+}
+
+int idisim_forward_noalloc(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n,
+                           double extent, void* out, int out_kind, void* stream)
+{
+    g_err.clear();
+    if (!h) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: NULL handle");
+    if (!out) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: NULL output");
+    if (n < 0 || (n > 0 && !emitters)) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: bad emitter arguments");
+    if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
+        return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: unknown output kind");
+    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: extent must be finite");
+    if (h->noalloc == nullptr) return fail(IDISIM_E_HIP, "idisim_forward_noalloc: no static scratch on this device");
+    return forward_call(h, emitters, amplitudes, n, extent, out, out_kind, static_cast<hipStream_t>(stream), true);
 }
 
 int idisim_forward_batched(idisim_handle* h, const double* emitters, const double* amplitudes,

@@ -126,6 +126,9 @@ typedef struct idisim_plan {
     int32_t reserved;
 } idisim_plan;
 
+/* Size of the static device scratch of idisim_forward_noalloc (per device). */
+#define IDISIM_NOALLOC_SCRATCH_BYTES (64u << 20)
+
 /* Library ABI version (IDISIM_VERSION). */
 IDISIM_API int idisim_version(void);
 
@@ -162,6 +165,22 @@ IDISIM_API int idisim_get_plan(const idisim_handle *h, idisim_plan *out);
                  float32 / float64 n0 x n1 for INTENSITY, row-major, ret[x, y] */
 IDISIM_API int idisim_forward(idisim_handle *h, const double *emitters, const double *amplitudes,
                    int64_t n, double extent, void *out, int out_kind, void *stream);
+
+/* idisim_forward without device allocation.  No counterpart in the reference: for
+   callers that cannot call hipMalloc (code under HIP graph capture, serving loops,
+   harnesses that forbid allocation).  The emitter-split partial sums (and the extent
+   word when extent <= 0) live in a static device buffer of
+   IDISIM_NOALLOC_SCRATCH_BYTES that each device holds for the life of the process;
+   the split count is capped so that its partials fit (n0 n1 x 16 B per split on
+   FP32-split handles, 32 B on FP64), and a call that needs more even at one split
+   (an FP64 fallback on a detector above 2^21 pixels) fails with IDISIM_E_INVALID.
+   The plan is the handle's tuned one (from earlier idisim_forward calls) or the
+   planner's heuristic; no timing pass runs.  Every call on a device shares that
+   device's buffer: calls that overlap in time on one device (different streams,
+   host threads or handles) race.  extent <= 0 is measured through the buffer and
+   synchronises `stream` (not allowed under graph capture: pass extent > 0). */
+IDISIM_API int idisim_forward_noalloc(idisim_handle *h, const double *emitters, const double *amplitudes,
+                                      int64_t n, double extent, void *out, int out_kind, void *stream);
 
 /* Many shots with uneven emitter counts in one persistent launch (replayed as
    a hipGraph when the buffers and counts repeat).  No reference counterpart;

@@ -23,7 +23,7 @@ def test_exports_every_declared_symbol(lib):
     from idi_amd import _lib
 
     names = _lib.declared_functions()
-    assert len(names) == 19
+    assert len(names) == 20
     for n in names:
         assert hasattr(lib, n), n
     assert lib.idisim_version() == _lib.ABI_VERSION == 2
@@ -58,6 +58,7 @@ def test_argument_errors_without_device(lib):
     g.k = -1.0
     assert lib.idisim_create(ctypes.byref(h), ctypes.byref(g), 1, 0) == -1
     assert lib.idisim_forward(None, None, None, 0, 0.0, None, 0, None) == -1
+    assert lib.idisim_forward_noalloc(None, None, None, 0, 0.0, None, 0, None) == -1
     assert lib.idisim_forward_batched(None, None, None, None, 1, 0.0, None, 0, None) == -1
     assert lib.idisim_adjoint(None, None, None, 0, 0.0, None, None, None, None, None, None) == -1
     assert lib.idisim_set_option(None, 1, 1) == -1
