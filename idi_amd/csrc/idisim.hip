@@ -15,6 +15,8 @@
 //     persistent batched launch.
 #include "../../include/idisim.h"
 #include "kernels.hpp"
+#include "checks.hpp"
+#include "planner.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -44,11 +46,6 @@ int fail(int code, const std::string& msg)
         if (e_ != hipSuccess) return fail(IDISIM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Largest error of v_sin_f32 / v_cos_f32 as an equivalent phase: measured on
-// MI355X against FP64 sin/cos over 4M arguments in [-64, 64] rev, max |dz| =
-// 1.29e-7 (2.05e-8 rev); with margin.
-constexpr double kSinCosErrRev = 3.0e-8;
-
 // A tuned choice is keyed by the call's shape and kernel family (FP32-split or
 // FP64); it is reused only while its tile is still among the eligible ones (a
 // larger emitter extent can push a tile's error bound past the tolerance).
@@ -74,24 +71,16 @@ struct Choice {
 
 }  // namespace
 
-struct idisim_handle {
+// The handle: the planner state (geometry, tolerance, tables; planner.hpp) and everything a
+// call needs besides.
+struct idisim_handle : idisim::Planner {
     int device = 0;
     int precision = IDISIM_PREC_FP32SPLIT;
     int num_cus = 256;
     idisim_geometry geom{};
-    Geom g{};
-    // options
-    int autotune = 1, force_tile = -1, force_split = 0, summode = 0, use_graph = 1, fallback = 1;
+    // options (tol_rev and summode are in the planner state)
+    int autotune = 1, force_tile = -1, force_split = 0, use_graph = 1, fallback = 1;
     int time_path = 0;
-    double tol_rev = 5e-6;
-    // planner tables
-    double du_max[kNumTiles] = {};
-    double dR_max[kNumTiles] = {};
-    double Rmin = 0.0, Rmax = 0.0;
-    double pair_coef = 0.0;   // mirror-pair remainder per unit emitter extent (rev per length)
-    double block_coef = 0.0;  // bound of |m_k - m_1| of the 2 x 4 block pairs per unit extent
-    double ref_coef = 0.0;    // bound of |d_k - D_k| (lane pair vs wave reference) per unit extent
-    double lin_coef = 0.0;    // largest residual of the linear wave reference (variant 13) per unit extent
     double rho = 0.0;   // emitter extent of the current call
     // device scratch
     void* scratch = nullptr;
@@ -134,215 +123,6 @@ struct DeviceGuard {
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 };
-
-void planner_tables(idisim_handle* h)
-{
-    const Geom& g = h->g;
-    h->Rmin = std::numeric_limits<double>::infinity();
-    h->Rmax = 0.0;
-    for (int v = 0; v < kNumTiles; ++v) {
-        const int TX = tile_rows(v), TY = tile_cols(v);
-        double du = 0.0, dR = 0.0;
-        for (int tx0 = 0; tx0 < g.n0; tx0 += TX) {
-            for (int ty0 = 0; ty0 < g.n1; ty0 += TY) {
-                const int xe = std::min(tx0 + TX, g.n0) - 1, ye = std::min(ty0 + TY, g.n1) - 1;
-                const Frame c = pixel_frame(g, 0.5 * (tx0 + xe), 0.5 * (ty0 + ye));
-                const double xs[3] = {(double)tx0, (double)((tx0 + xe) / 2), (double)xe};
-                const double ys[3] = {(double)ty0, (double)((ty0 + ye) / 2), (double)ye};
-                h->Rmin = std::min(h->Rmin, c.R);
-                h->Rmax = std::max(h->Rmax, c.R);
-                for (double fx : xs) {
-                    for (double fy : ys) {
-                        const Frame f = pixel_frame(g, fx, fy);
-                        const double d0 = f.u[0] - c.u[0], d1 = f.u[1] - c.u[1], d2 = f.u[2] - c.u[2];
-                        du = std::max(du, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
-                        dR = std::max(dR, std::fabs(f.R - c.R));
-                        h->Rmin = std::min(h->Rmin, f.R);
-                        h->Rmax = std::max(h->Rmax, f.R);
-                    }
-                }
-            }
-        }
-        // corners and edge midpoints sample the tile; u is smooth, margin 5%
-        h->du_max[v] = 1.05 * du;
-        h->dR_max[v] = 1.05 * dR;
-    }
-    // Mirror-pair remainder: for a 2 x 2 quad the mean dq of a pair differs from the
-    // quad mean by 1/4 of the mixed second difference, krev pix^2 |d_a d_b rh| / 4
-    // (plus fourth-order terms).  With orthonormal detector axes a, b and
-    // alpha = rh . a, beta = rh . b:  d_a d_b rh = (3 alpha beta rh - alpha b - beta a) / R^2,
-    // so |d_a d_b rh| <= (sqrt(2) s + 3 s^2) / R^2 with s >= |alpha|, |beta|.  In the
-    // detector frame alpha = dx / R and beta = dy / R (dx, dy the in-plane pixel
-    // coordinates), largest in magnitude at the corners; one pixel is added for the
-    // quads that reach past the last row or column.
-    double emax = 0.0;
-    for (double fx : {-1.0, (double)g.n0}) {
-        for (double fy : {-1.0, (double)g.n1}) {
-            emax = std::max(emax, std::fabs((fx - g.half0) * g.pix + g.off0));
-            emax = std::max(emax, std::fabs((fy - g.half1) * g.pix + g.off1));
-        }
-    }
-    const double sm = std::min(1.0, emax / h->Rmin);
-    h->pair_coef = 1.05 * g.krev * g.pix * g.pix * (std::sqrt(2.0) * sm + 3.0 * sm * sm) /
-                   (4.0 * h->Rmin * h->Rmin);
-    // 2 x 4 block pairs: m_k - m_1 = (D2f[o_k] - D2f[o_1]) / 2 + O(o^4), where
-    // |D2f[o]| <= krev pix^2 |o|^2 (1 + (2 + sqrt 2) s + 6 s^2) / R^2 from
-    // d_a d_a rh = (3 alpha^2 rh - rh - 2 alpha a) / R^2 and the mixed derivative
-    // above; |o_k|^2 <= 5/2, |o_1|^2 = 1/2.  Blocks reach three columns past the
-    // last one.
-    double ebl = emax;
-    for (double fy : {-1.0, (double)g.n1 + 2.0})
-        ebl = std::max(ebl, std::fabs((fy - g.half1) * g.pix + g.off1));
-    const double sb = std::min(1.0, ebl / h->Rmin);
-    h->block_coef = 1.05 * 1.5 * (1.0 + (2.0 + std::sqrt(2.0)) * sb + 6.0 * sb * sb) * g.krev * g.pix * g.pix /
-                    (h->Rmin * h->Rmin);
-    // Wave references (variants 9..12): d_k(p) - D_k = int_0^1 int_-1^1 D2f[a_k, o] dt ds along the
-    // segment from the block centre P to the quad centre p, o = p - P, |a_k| = sqrt(2) / 2 pixel.
-    // D2f is a symmetric bilinear form with |D2f[v, v]| <= krev pix^2 |v|^2 C(s) / R^2,
-    // C(s) = 1 + (2 + sqrt 2) s + 6 s^2 (as above), so |D2f[a, o]| <= krev pix^2 |a| |o| C(s) / R^2.
-    // In a 16 x 16 pixel wave block |o| <= 7 sqrt 2 pixels.  The segments and the block centres
-    // reach to the edge of the last (padded) 32 x 32 tile, so s and R are taken over the padded
-    // detector, R at its point nearest the foot of the detector normal.
-    {
-        const double x0 = (-1.0 - g.half0) * g.pix + g.off0;
-        const double x1 = (32.0 * ((g.n0 + 31) / 32) - g.half0) * g.pix + g.off0;
-        const double y0 = (-1.0 - g.half1) * g.pix + g.off1;
-        const double y1 = (32.0 * ((g.n1 + 31) / 32) - g.half1) * g.pix + g.off1;
-        const double ex = std::max(std::fabs(x0), std::fabs(x1)), ey = std::max(std::fabs(y0), std::fabs(y1));
-        const double nx = std::min(std::max(0.0, x0), x1), ny = std::min(std::max(0.0, y0), y1);
-        const double Rp = std::min(h->Rmin, std::sqrt(nx * nx + ny * ny + g.detz * g.detz));
-        const double sr = std::min(1.0, std::max(ex, ey) / Rp);
-        h->ref_coef = 1.05 * (1.0 + (2.0 + std::sqrt(2.0)) * sr + 6.0 * sr * sr) * g.krev * g.pix * g.pix *
-                      (0.5 * std::sqrt(2.0)) * (7.0 * std::sqrt(2.0)) / (Rp * Rp);
-    }
-    // Linear wave reference (variant 13): the kernel rotates by (ox Jx_k + oy Jy_k) . r' in place
-    // of (d_k - D_k) . r'.  The residual d_k - D_k - ox Jx_k - oy Jy_k is evaluated here for every
-    // quad that holds a detector pixel, with the FP32 D, Jx, Jy of the kernel (the same
-    // wave_linear_ref), and its largest norm (with 5% margin for the FP64 rounding of this
-    // evaluation) enters the bound.  Detectors beyond 2048 x 2048 pixels skip the table and the
-    // variant.
-    h->lin_coef = std::numeric_limits<double>::infinity();
-    if ((int64_t)g.n0 * g.n1 <= (int64_t)2048 * 2048) {
-        double worst = 0.0;
-        for (int tx0 = 0; tx0 < g.n0; tx0 += 32) {
-            for (int ty0 = 0; ty0 < g.n1; ty0 += 32) {
-                const int xe = std::min(tx0 + 32, g.n0) - 1, ye = std::min(ty0 + 32, g.n1) - 1;
-                const Frame c = pixel_frame(g, 0.5 * (double)(tx0 + xe), 0.5 * (double)(ty0 + ye));
-                for (int w = 0; w < 4; ++w) {
-                    const int bx = tx0 + 16 * (w >> 1), by = ty0 + 16 * (w & 1);
-                    if (bx >= g.n0 || by >= g.n1) continue;
-                    float D[2][3], Jx[2][3], Jy[2][3];
-                    wave_linear_ref(g, c.u, bx + 7.5, by + 7.5, D, Jx, Jy);
-                    for (int i = 0; i < 8 && bx + 2 * i < g.n0; ++i) {
-                        for (int j = 0; j < 8 && by + 2 * j < g.n1; ++j) {
-                            double hq[2][3];
-                            quad_half_diff(g, c.u, (double)(bx + 2 * i), (double)(by + 2 * j), hq);
-                            const double ox = 2 * i - 7, oy = 2 * j - 7;
-                            for (int k = 0; k < 2; ++k) {
-                                double n2 = 0.0;
-                                for (int a = 0; a < 3; ++a) {
-                                    const double r = hq[k][a] - (double)D[k][a] - ox * (double)Jx[k][a] -
-                                                     oy * (double)Jy[k][a];
-                                    n2 += r * r;
-                                }
-                                worst = std::max(worst, std::sqrt(n2));
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        h->lin_coef = 1.05 * worst;
-    }
-}
-
-// Error (revolutions) of folding W_cj into A_cj, i.e. of replacing lam_p W_cj
-// by W_cj: W_max max|lam_p - 1|.
-double fold_error(const idisim_handle* h, int v, double rho)
-{
-    const Geom& g = h->g;
-    if (!g.nf && !g.secondorder) return 0.0;
-    const double dR = h->dR_max[v];
-    const double Rlo = h->Rmin - dR - rho;
-    if (!(Rlo > 0.0)) return std::numeric_limits<double>::infinity();
-    const double W = g.krev * rho * rho / (2.0 * Rlo);
-    return W * dR / (h->Rmin - dR);
-}
-
-// A-priori bound of the per-term phase error (revolutions) of tile variant v
-// for emitters within |r_j| <= rho.
-double plan_bound(const idisim_handle* h, int v, double rho)
-{
-    const Geom& g = h->g;
-    const double du = h->du_max[v], dR = h->dR_max[v];
-    const double Rlo = h->Rmin - dR - rho;
-    if (!(Rlo > 0.0)) return std::numeric_limits<double>::infinity();
-    const double Rhi = h->Rmax + dR + rho;
-    const double ds = du * rho;
-    double wmax = 0.0, sig = 0.0;
-    if (g.nf) {
-        wmax = rho * rho / (2.0 * Rlo);
-        sig = (rho + wmax) / Rlo;
-    } else if (g.secondorder) {
-        wmax = rho * rho / (2.0 * Rlo);
-    }
-    const double W = g.krev * wmax;
-    const double B = g.krev * ds * (1.0 + sig);
-    const double tau = 0.5 + W + B;
-    const double ulp = std::ldexp(1.0, std::ilogb(tau) - 23);
-    const double u24 = std::ldexp(1.0, -24);
-    double e = 2.0 * ulp;                // four FMA roundings of tau, each <= ulp/2
-    e += 2.0 * u24 * B;                  // dq and r' rounded to FP32: sum_i |dq_i r_i| <= B
-    e += 0.5 * u24 + 6.0 * u24 * W;      // A rounded; W formed in FP32; lam rounded
-    e += 4.0 * u24 * B;                  // r' = r (1 + sigma) formed in FP32
-    if (g.nf) {
-        const double R3 = Rlo * Rlo * Rlo;
-        const double wss = Rhi * Rhi / R3;
-        const double wsR = (Rhi * rho + rho * rho) / R3;
-        const double wRR = 2.0 * wmax * Rhi / R3;
-        const double lin = 0.5 * wss * ds * ds + wsR * ds * dR + 0.5 * wRR * dR * dR +
-                           wmax * dR * (dR + rho) / (Rlo * Rlo);
-        e += g.krev * lin;
-        if (g.scale) e += (ds / Rlo + rho * dR / (Rlo * Rlo)) * kInv2Pi;  // amplitude, as phase
-    }
-    e += kSinCosErrRev;
-    const double fe = fold_error(h, v, rho);
-    if (fe <= 0.1 * h->tol_rev) {
-        e += fe;  // W folded into A
-    } else if (tile_pair(v) && (g.nf || g.secondorder)) {
-        return std::numeric_limits<double>::infinity();  // mirror pairs need the fold
-    }
-    if (tile_pair(v)) {
-        const double rem = h->pair_coef * rho * (1.0 + sig);  // |e_k| = |(m_k - mc) . r'|
-        if (v == kTileBlock) {
-            // the three rotated centres, first order: |Delta| <= 2 pi block_coef rho (1 + sigma)
-            const double delta = 2.0 * M_PI * h->block_coef * rho * (1.0 + sig);
-            e += (0.5 * delta * delta + 4.0 * u24 * delta) * kInv2Pi;
-        } else if (tile_pair_centred(v)) {
-            // pair 1's centre phasor to first order: |Delta| = 2 pi |(m_0 - m_1) . r'| <= 4 pi rem,
-            // error Delta^2 / 2 (rad), plus Delta's own FP32 rounding
-            const double delta = 4.0 * M_PI * rem;
-            e += (0.5 * delta * delta + 4.0 * u24 * delta) * kInv2Pi;
-        } else {
-            e += rem;  // dropped remainder e_k
-        }
-        e += kSinCosErrRev;  // second sincos (delta_k, or the reference E_k)
-        e += 2.0 * u24;      // e^{i tau_c} (cos, sin) delta_k products, as phase
-        if (const int ord = tile_ref_order(v)) {
-            // rotation from the wave reference, t = 2 pi (d_k - D_k) . r': truncation t^2 / 2
-            // (first order) or |t|^3 / 6 (second order), t's own FP32 rounding, and the
-            // rounding of D_k and of the rotated (cos, sin).  The linear reference (variant 13)
-            // rotates by the linear part of d_k - D_k instead: its residual enters as a phase
-            // error of its own, and its slopes are formed and combined in FP32 (8 roundings).
-            const bool lin = tile_ref_kind(v) == 3;
-            const double t = 2.0 * M_PI * (h->ref_coef + (lin ? h->lin_coef : 0.0)) * rho * (1.0 + sig);
-            const double trunc = ord == 1 ? 0.5 * t * t : t * t * t / 6.0;
-            e += (trunc + (lin ? 8.0 : 4.0) * u24 * t) * kInv2Pi + 2.0 * u24 * B + 2.0 * u24;
-            if (lin) e += h->lin_coef * rho * (1.0 + sig);
-        }
-    }
-    return e;
-}
 
 size_t out_elem_bytes(int precision, int kind)
 {
@@ -398,20 +178,6 @@ int nbucket(int64_t n)
         ++b;
     }
     return b;
-}
-
-// Tile variants of the FP32-split kernels whose error bound meets the tolerance.
-std::vector<int> eligible_tiles(const idisim_handle* h, double rho, double* best_bound)
-{
-    std::vector<int> out;
-    *best_bound = std::numeric_limits<double>::infinity();
-    for (int v = 0; v < kNumTiles; ++v) {
-        if (v == kTileNone || (tile_pair(v) && h->summode != 0)) continue;
-        const double b = plan_bound(h, v, rho);
-        *best_bound = std::min(*best_bound, b);
-        if (b <= h->tol_rev) out.push_back(v);
-    }
-    return out;
 }
 
 struct Timer {
@@ -861,16 +627,10 @@ const char* idisim_last_error(void) { return g_err.c_str(); }
 int idisim_create(idisim_handle** out, const idisim_geometry* geom, int precision, int device)
 {
     g_err.clear();
-    if (!out || !geom) return fail(IDISIM_E_INVALID, "idisim_create: NULL argument");
+    if (!out) return fail(IDISIM_E_INVALID, "idisim_create: NULL argument");
     *out = nullptr;
-    if (geom->n0 <= 0 || geom->n1 <= 0 || geom->n0 > (1 << 20) || geom->n1 > (1 << 20) ||
-        geom->n0 * geom->n1 > (int64_t)1 << 31)
-        return fail(IDISIM_E_INVALID, "idisim_create: detector size out of range");
-    if (!(geom->k > 0.0) || !std::isfinite(geom->k) || !std::isfinite(geom->pixelsize) ||
-        !std::isfinite(geom->detz))
-        return fail(IDISIM_E_INVALID, "idisim_create: k, pixelsize and detz must be finite, k > 0");
-    if (precision != IDISIM_PREC_FP64 && precision != IDISIM_PREC_FP32SPLIT)
-        return fail(IDISIM_E_INVALID, "idisim_create: unknown precision");
+    std::string msg;
+    if (int rc = check_create(geom, precision, &msg)) return fail(rc, msg);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(IDISIM_E_HIP, "idisim_create: no HIP device available");
@@ -883,22 +643,7 @@ int idisim_create(idisim_handle** out, const idisim_geometry* geom, int precisio
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         h->num_cus = prop.multiProcessorCount;
-    Geom& g = h->g;
-    g.pix = geom->pixelsize;
-    g.detz = geom->detz;
-    g.k = geom->k;
-    g.krev = geom->k * kInv2Pi;
-    g.off0 = geom->detoffset[0];
-    g.off1 = geom->detoffset[1];
-    g.half0 = (double)geom->n0 / 2.0;
-    g.half1 = (double)geom->n1 / 2.0;
-    for (int i = 0; i < 9; ++i) g.M[i] = geom->rot[i];
-    g.n0 = (int)geom->n0;
-    g.n1 = (int)geom->n1;
-    g.rot = (geom->flags & IDISIM_ROTATE) ? 1 : 0;
-    g.nf = (geom->flags & IDISIM_NF) ? 1 : 0;
-    g.scale = (geom->flags & IDISIM_SCALE) ? 1 : 0;
-    g.secondorder = (geom->flags & IDISIM_SECONDORDER) ? 1 : 0;
+    h->g = make_geom(*geom);
     planner_tables(h);
     h->plan.precision = precision;
     if (hipHostMalloc(reinterpret_cast<void**>(&h->h_viol), sizeof(unsigned int),
@@ -945,32 +690,18 @@ int idisim_destroy(idisim_handle* h)
 int idisim_set_option(idisim_handle* h, int option, int64_t value)
 {
     if (!h) return fail(IDISIM_E_INVALID, "idisim_set_option: NULL handle");
+    std::string msg;
+    if (int rc = check_option(option, value, kNumTiles, &msg)) return fail(rc, msg);
     switch (option) {
         case IDISIM_OPT_AUTOTUNE: h->autotune = value ? 1 : 0; break;
-        case IDISIM_OPT_TILE:
-            if (value < -1 || value >= kNumTiles) return fail(IDISIM_E_INVALID, "tile variant out of range");
-            if (value == kTileNone) return fail(IDISIM_E_UNSUPPORTED, "tile variant 4 has no kernel");
-            h->force_tile = (int)value;
-            break;
-        case IDISIM_OPT_SPLIT:
-            if (value < 0 || value > 4096) return fail(IDISIM_E_INVALID, "split out of range");
-            h->force_split = (int)value;
-            break;
-        case IDISIM_OPT_SUMMODE:
-            if (value != 0 && value != 1) return fail(IDISIM_E_INVALID, "summode must be 0 or 1");
-            h->summode = (int)value;
-            break;
+        case IDISIM_OPT_TILE: h->force_tile = (int)value; break;
+        case IDISIM_OPT_SPLIT: h->force_split = (int)value; break;
+        case IDISIM_OPT_SUMMODE: h->summode = (int)value; break;
         case IDISIM_OPT_GRAPH: h->use_graph = value ? 1 : 0; break;
-        case IDISIM_OPT_TOL_PPB:
-            if (value <= 0) return fail(IDISIM_E_INVALID, "tolerance must be positive");
-            h->tol_rev = (double)value * 1e-9;
-            break;
+        case IDISIM_OPT_TOL_PPB: h->tol_rev = (double)value * 1e-9; break;
         case IDISIM_OPT_FALLBACK: h->fallback = value ? 1 : 0; break;
-        case IDISIM_OPT_TIME_PATH:
-            if (value < 0 || value > 2) return fail(IDISIM_E_INVALID, "IDISIM_OPT_TIME_PATH must be 0, 1 or 2");
-            h->time_path = (int)value;
-            break;
-        default: return fail(IDISIM_E_INVALID, "unknown option " + std::to_string(option));
+        case IDISIM_OPT_TIME_PATH: h->time_path = (int)value; break;
+        default: break;
     }
     h->tuned.clear();
     return IDISIM_OK;
@@ -1011,9 +742,8 @@ int idisim_generate_ball(uint64_t seed, uint64_t rng_stream, int64_t n, double r
                          int positions, int phases, double* out, void* stream)
 {
     g_err.clear();
-    if (n < 0 || (n > 0 && !out)) return fail(IDISIM_E_INVALID, "idisim_generate_ball: bad arguments");
-    if (!(radius >= 0.0) || !std::isfinite(radius))
-        return fail(IDISIM_E_INVALID, "idisim_generate_ball: radius must be finite and >= 0");
+    std::string msg;
+    if (int rc = check_generate("idisim_generate_ball", n, out, radius, &msg)) return fail(rc, msg);
     if (centre && !(std::isfinite(centre[0]) && std::isfinite(centre[1]) && std::isfinite(centre[2])))
         return fail(IDISIM_E_INVALID, "idisim_generate_ball: centre must be finite");
     IDISIM_HIP(launch_gen_ball(seed, rng_stream, n, radius, centre, positions ? 1 : 0, phases ? 1 : 0, out,
@@ -1026,11 +756,10 @@ int idisim_generate_balls(uint64_t seed, uint64_t rng_stream, int64_t n, double 
                           void* stream)
 {
     g_err.clear();
-    if (n < 0 || (n > 0 && !out)) return fail(IDISIM_E_INVALID, "idisim_generate_balls: bad arguments");
+    std::string msg;
+    if (int rc = check_generate("idisim_generate_balls", n, out, radius, &msg)) return fail(rc, msg);
     if (positions && n > 0 && (nballs < 1 || !centres || !offsets))
         return fail(IDISIM_E_INVALID, "idisim_generate_balls: positions need nballs >= 1, centres and offsets");
-    if (!(radius >= 0.0) || !std::isfinite(radius))
-        return fail(IDISIM_E_INVALID, "idisim_generate_balls: radius must be finite and >= 0");
     IDISIM_HIP(launch_gen_balls(seed, rng_stream, n, radius, nballs, centres, offsets, positions ? 1 : 0,
                                 phases ? 1 : 0, out, static_cast<hipStream_t>(stream)));
     return IDISIM_OK;
@@ -1059,11 +788,8 @@ int idisim_time_forward(idisim_handle* h, const double* emitters, const double* 
                         double tau, double* out, void* stream)
 {
     g_err.clear();
-    if (!h) return fail(IDISIM_E_INVALID, "idisim_time_forward: NULL handle");
-    if (n < 0 || n >= ((int64_t)1 << 31) || (n > 0 && (!emitters || !times)) || !out)
-        return fail(IDISIM_E_INVALID, "idisim_time_forward: bad arguments");
-    if (!(c > 0.0) || !std::isfinite(c) || !(tau > 0.0) || !std::isfinite(tau))
-        return fail(IDISIM_E_INVALID, "idisim_time_forward: c and tau must be finite and > 0");
+    std::string msg;
+    if (int rc = check_time_forward(h, emitters, times, n, c, tau, out, &msg)) return fail(rc, msg);
     DeviceGuard guard(h->device);
     const TimeLayout L = time_layout(h->g, n, h->precision == IDISIM_PREC_FP64, (size_t)1 << 30);
     if (n > 0 && !time_fused_ok(n, L.fp64, h->time_path)) {
@@ -1089,13 +815,8 @@ int idisim_qcorr(const float* zd, const float* img, int64_t nr, int64_t nc, doub
                  int64_t qmax, int64_t maxdqz, double* out, void* stream)
 {
     g_err.clear();
-    if (!zd || !img || !out) return fail(IDISIM_E_INVALID, "idisim_qcorr: NULL buffer");
-    if (nr < 1 || nc < 1 || nr > 65535 || nc > (1 << 20))
-        return fail(IDISIM_E_INVALID, "idisim_qcorr: image size out of range");
-    if (qmax < 1 || qmax > 1024 || maxdqz < 1 || maxdqz > (1 << 20))
-        return fail(IDISIM_E_INVALID, "idisim_qcorr: qmax must be in [1, 1024] and maxdqz >= 1");
-    if (!std::isfinite(xcenter) || !std::isfinite(ycenter) || !std::isfinite(z))
-        return fail(IDISIM_E_INVALID, "idisim_qcorr: centre and z must be finite");
+    std::string msg;
+    if (int rc = check_qcorr(zd, img, nr, nc, xcenter, ycenter, z, qmax, maxdqz, out, &msg)) return fail(rc, msg);
     IDISIM_HIP(launch_qcorr(zd, img, (int)nr, (int)nc, (float)xcenter, (float)ycenter, (float)z, (int)qmax,
                             (int)maxdqz, out, static_cast<hipStream_t>(stream)));
     return IDISIM_OK;
@@ -1131,17 +852,8 @@ int idisim_chargesharing(const double* counts, int64_t n0, int64_t n1, double si
                          uint64_t seed, double* out, void* stream)
 {
     g_err.clear();
-    if (n0 <= 0 || n1 <= 0 || n0 > (1 << 20) || n1 > (1 << 20) || n0 * n1 > ((int64_t)1 << 31))
-        return fail(IDISIM_E_INVALID, "idisim_chargesharing: image size out of range");
-    if (!counts || !out) return fail(IDISIM_E_INVALID, "idisim_chargesharing: NULL buffer");
-    const size_t bytes = (size_t)(n0 * n1) * sizeof(double);
-    const char* a = reinterpret_cast<const char*>(counts);
-    const char* b = reinterpret_cast<const char*>(out);
-    if (a < b + bytes && b < a + bytes) return fail(IDISIM_E_INVALID, "idisim_chargesharing: out overlaps counts");
-    if (sigma > 32.0)
-        return fail(IDISIM_E_INVALID, "idisim_chargesharing: sigma is limited to 32 pixels");
-    if (!std::isfinite(gain_noise) || gain_noise < 0.0)
-        return fail(IDISIM_E_INVALID, "idisim_chargesharing: gain_noise must be finite and >= 0");
+    std::string msg;
+    if (int rc = check_chargesharing(counts, n0, n1, sigma, gain_noise, out, &msg)) return fail(rc, msg);
     IDISIM_HIP(launch_chargesharing(counts, (int)n0, (int)n1, sigma, gain_noise, seed, out,
                                     static_cast<hipStream_t>(stream)));
     return IDISIM_OK;
@@ -1226,12 +938,8 @@ int idisim_forward(idisim_handle* h, const double* emitters, const double* ampli
                    void* out, int out_kind, void* stream)
 {
     g_err.clear();
-    if (!h) return fail(IDISIM_E_INVALID, "idisim_forward: NULL handle");
-    if (!out) return fail(IDISIM_E_INVALID, "idisim_forward: NULL output");
-    if (n < 0 || (n > 0 && !emitters)) return fail(IDISIM_E_INVALID, "idisim_forward: bad emitter arguments");
-    if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
-        return fail(IDISIM_E_INVALID, "idisim_forward: unknown output kind");
-    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward: extent must be finite");
+    std::string msg;
+    if (int rc = check_forward("idisim_forward", h, emitters, n, out, out_kind, extent, &msg)) return fail(rc, msg);
     return forward_call(h, emitters, amplitudes, n, extent, out, out_kind, static_cast<hipStream_t>(stream), false);
 }
 
@@ -1239,12 +947,9 @@ int idisim_forward_noalloc(idisim_handle* h, const double* emitters, const doubl
                            double extent, void* out, int out_kind, void* stream)
 {
     g_err.clear();
-    if (!h) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: NULL handle");
-    if (!out) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: NULL output");
-    if (n < 0 || (n > 0 && !emitters)) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: bad emitter arguments");
-    if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
-        return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: unknown output kind");
-    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward_noalloc: extent must be finite");
+    std::string msg;
+    if (int rc = check_forward("idisim_forward_noalloc", h, emitters, n, out, out_kind, extent, &msg))
+        return fail(rc, msg);
     if (h->noalloc == nullptr) return fail(IDISIM_E_HIP, "idisim_forward_noalloc: no static scratch on this device");
     return forward_call(h, emitters, amplitudes, n, extent, out, out_kind, static_cast<hipStream_t>(stream), true);
 }
@@ -1254,17 +959,9 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
                            void* stream)
 {
     g_err.clear();
-    if (!h) return fail(IDISIM_E_INVALID, "idisim_forward_batched: NULL handle");
-    if (!out || !offsets || nshots <= 0) return fail(IDISIM_E_INVALID, "idisim_forward_batched: bad arguments");
-    if (out_kind != IDISIM_OUT_FIELD && out_kind != IDISIM_OUT_INTENSITY)
-        return fail(IDISIM_E_INVALID, "idisim_forward_batched: unknown output kind");
-    if (offsets[0] != 0) return fail(IDISIM_E_INVALID, "idisim_forward_batched: offsets[0] must be 0");
-    for (int64_t s = 0; s < nshots; ++s)
-        if (offsets[s + 1] < offsets[s])
-            return fail(IDISIM_E_INVALID, "idisim_forward_batched: offsets must be non-decreasing");
+    std::string msg;
+    if (int rc = check_batched(h, emitters, offsets, nshots, out, out_kind, extent, &msg)) return fail(rc, msg);
     const int64_t total = offsets[nshots];
-    if (total > 0 && !emitters) return fail(IDISIM_E_INVALID, "idisim_forward_batched: NULL emitters");
-    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_forward_batched: extent must be finite");
     DeviceGuard guard(h->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (int rc0 = check_extent_flag(h, "idisim_forward_batched")) return rc0;
@@ -1385,12 +1082,10 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
                    void* stream)
 {
     g_err.clear();
-    if (!h) return fail(IDISIM_E_INVALID, "idisim_adjoint: NULL handle");
-    if (n < 0 || (n > 0 && (!emitters || !grad_pos || !grad_phase)) || !field || !dLdI)
-        return fail(IDISIM_E_INVALID, "idisim_adjoint: bad arguments");
+    std::string msg;
+    if (int rc = check_adjoint(h, emitters, n, field, dLdI, grad_pos, grad_phase, extent, &msg)) return fail(rc, msg);
     if (h->precision != IDISIM_PREC_FP32SPLIT)
         return fail(IDISIM_E_UNSUPPORTED, "idisim_adjoint: only FP32-split handles have an adjoint");
-    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, "idisim_adjoint: extent must be finite");
     if (n == 0) return IDISIM_OK;
     DeviceGuard guard(h->device);
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -1,0 +1,132 @@
+"""The host side of the C ABI under AddressSanitizer and UndefinedBehaviorSanitizer (no GPU).
+
+tests/host_asan/planner_check.cpp compiles the FP32-split error planner (idi_amd/csrc/planner.hpp)
+and the entry points' argument checks (idi_amd/csrc/checks.hpp), the same headers the library
+compiles, with g++ -fsanitize=address,undefined -fno-sanitize-recover=all.  Any memory error or
+undefined behaviour aborts the driver, and the test fails.  Checked here:
+  * every argument check returns the documented code on good and bad arguments;
+  * the planner's tables on the detectors of tests/test_planner_cpu.py equal that file's NumPy
+    restatement of the same coefficients (mirror-pair remainder, 2 x 4 block spread, wave
+    reference offset, linear-reference residual);
+  * in the far field the planner's bounds grow with the extent; the eligible tiles are exactly those within the
+    tolerance (none of the mirror-pair tiles with per-term Kahan sums, never the reserved index 4),
+    and the bound of the linear-reference tile on the 100k-site image equals the one the library
+    reported on the GPU for that workload.
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import K_UM
+from test_planner_cpu import CASES, _block_bound_coef, _bound_coef, _lin_residual, _ref_bound_coef
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TOL = 5e-6
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    if shutil.which("g++") is None or not os.path.exists("/opt/rocm/include/hip/hip_runtime.h"):
+        pytest.skip("needs g++ and the ROCm headers")
+    build = tmp_path_factory.mktemp("host_asan")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "host_asan"), f"BUILD={build}"], check=True)
+    return str(build / "planner_check")
+
+
+def _env():
+    env = dict(os.environ)
+    env["ASAN_OPTIONS"] = "detect_leaks=1:halt_on_error=1:abort_on_error=1"
+    env["UBSAN_OPTIONS"] = "halt_on_error=1:print_stacktrace=1"
+    return env
+
+
+def _plan(driver, geoms, rhos, summode=0):
+    lines = []
+    for g in geoms:
+        M = np.asarray(g.M, dtype=np.float64).ravel()
+        lines.append(" ".join(str(v) for v in (
+            g.n0, g.n1, repr(float(g.pixelsize)), repr(float(g.detz)), repr(float(g.k)), repr(float(g.detoffset[0])),
+            repr(float(g.detoffset[1])), int(g.nf), int(g.scale), int(g.secondorder), int(g.rot), *(repr(float(v)) for v in M),
+            TOL, summode, len(rhos), *(repr(float(v)) for v in rhos))))
+    out = subprocess.run([driver], input="\n".join(lines) + "\n", capture_output=True, text=True, env=_env())
+    assert out.returncode == 0, out.stderr[-3000:]
+    return [json.loads(s) for s in out.stdout.splitlines()]
+
+
+def test_argument_checks(driver):
+    out = subprocess.run([driver, "checks"], capture_output=True, text=True, env=_env())
+    assert out.returncode == 0, out.stderr[-3000:]
+    codes = json.loads(out.stdout)
+    assert len(codes) >= 30
+    for name, code in codes.items():
+        if name.endswith("_ok"):
+            assert code == 0, name
+        elif name == "option_tile_reserved":
+            assert code == -5, name  # IDISIM_E_UNSUPPORTED
+        else:
+            assert code == -1, name  # IDISIM_E_INVALID
+
+
+def _geom(args, nf=False):
+    from oracle import Geometry
+
+    return Geometry(args["n0"], args["n1"], args["pixelsize"], args["detz"], K_UM, nf=nf,
+                    detoffset=args.get("detoffset", (0.0, 0.0)), detangles=args.get("detangles", (0.0, 0.0)))
+
+
+def test_planner_tables_match_the_numpy_restatement(driver):
+    geoms = [_geom(a) for a, _ in CASES]
+    res = _plan(driver, geoms, [0.01])
+    for (args, _), g, r in zip(CASES, geoms, res):
+        # the compiled planner takes R_min over the pixel range its tiles reach (one pixel before the
+        # first, to the end of the last 32 x 32 tile), the restatement over the detector's pixels: the
+        # same on the centred detectors, no larger elsewhere, so the compiled coefficients are at
+        # least the restatement's
+        for key, ref in (("pair_coef", _bound_coef(g)), ("block_coef", _block_bound_coef(g)),
+                         ("ref_coef", _ref_bound_coef(g))):
+            if not any(args.get(k) for k in ("detoffset", "detangles")):
+                assert r[key] == pytest.approx(ref, rel=1e-12), (args, key)
+            assert ref * (1 - 1e-12) <= r[key] <= 1.2 * ref, (args, key, r[key], ref)
+    small = [_geom(a) for a, _ in CASES if a["n0"] * a["n1"] <= 64 * 64]
+    for g, r in zip(small, _plan(driver, small, [0.01])):
+        # the FP32 roundings of D and J may fall differently (FP64 formed in another order)
+        assert r["lin_coef"] == pytest.approx(1.05 * _lin_residual(g), rel=1e-4)
+
+
+@pytest.mark.parametrize("nf", [False, True])
+def test_planner_bounds_and_eligibility(driver, nf):
+    rhos = [0.002, 0.005, 0.01, 0.02, 0.03, 0.05, 0.1]
+    geoms = [_geom(a, nf=nf) for a, _ in CASES]
+    for summode in (0, 1):
+        for r in _plan(driver, geoms, rhos, summode):
+            prev = None
+            for e in r["extents"]:
+                b = [np.inf if v is None else v for v in e["bound"]]
+                want = [v for v in range(14) if v != 4 and b[v] <= TOL and not (summode and v >= 5)]
+                assert e["eligible"] == want, (e["rho"], e["eligible"], want)
+                if prev is not None and not nf:
+                    # no curvature term to fold: a larger extent gives a larger bound (in the near
+                    # field the bound drops where W stops being folded and its fold error with it)
+                    assert all(x >= y for x, y in zip(b, prev)), e["rho"]
+                prev = b
+
+
+def test_planner_bound_of_the_benchmark_image(driver):
+    """The bound that the library's plan reported on the GPU for the 100k-site image (tile 13,
+    extent = max |r_j| of the benchmark's emitters) is what the host build computes."""
+    from oracle import Geometry
+
+    rng = np.random.default_rng(2000)
+    m = int(np.ceil(0.01 / 3e-4))
+    grid = np.arange(-m, m + 1) * 3e-4
+    pts = np.stack(np.meshgrid(grid, grid, grid, indexing="ij"), -1).reshape(-1, 3)
+    pts = pts[np.linalg.norm(pts, axis=1) <= 0.01 * (1 + 1e-12)]
+    sel = pts[rng.choice(len(pts), 100_000, replace=False)]
+    rho = float(np.linalg.norm(sel, axis=1).max())
+    g = Geometry(512, 512, 75.0, 2e5, K_UM, nf=True)
+    (r,) = _plan(driver, [g], [rho])
+    assert r["extents"][0]["bound"][13] == pytest.approx(1.6306114528647848e-06, rel=1e-12)
