@@ -455,16 +455,18 @@ def bench_adjoint(args):
     target32 = target.float()
 
     def step():
+        # the forward kernel, then one adjoint pass that forms dL/dI = 2 (I - T) of
+        # L = sum (I - T)^2 itself and returns L with the gradients
         h.forward(d, None, field, _lib.OUT_FIELD, ext)
-        I = torch.view_as_real(field).square().sum(-1)
-        dLdI = torch.sub(I, target32).mul_(2.0)  # dL/dI of L = sum (I - T)^2
-        h.adjoint(d, field, dLdI, None, ext, want_amp=False)
+        return h.adjoint_l2(d, field, target32, None, ext, want_amp=False)
 
     step()
     torch.cuda.synchronize()
     timer = StepTimer()
     timer.run(step, args.warmup)
     wall, ms = timer.run(step, args.steps)
+    h.forward(d, None, field, _lib.OUT_FIELD, ext)
+    fwd_plan = h.plan()
     # the gradients of one step against long-double central differences of the loss (h = 1e-4 nm for
     # positions, 1e-4 rad for phases) on 24 position and 8 phase coordinates, and against the FP64
     # analytic gradient of the oracle over all 4N coordinates
@@ -473,11 +475,10 @@ def bench_adjoint(args):
         import oracle
 
         geom = oracle.Geometry(n0, n1, 75.0, 1e5, K, nf=True)
-        h.forward(d, None, field, _lib.OUT_FIELD, ext)
-        I = torch.view_as_real(field).square().sum(-1)
-        gpos, gph, _ = h.adjoint(d, field, torch.sub(I, target32).mul_(2.0), None, ext, want_amp=False)
+        loss, gpos, gph, _ = step()
+        loss = float(loss)
         gpos, gph = gpos.cpu().numpy().astype(np.float64), gph.cpu().numpy().astype(np.float64)
-        T = target.cpu().numpy()
+        T = target32.double().cpu().numpy()
         crng = np.random.default_rng(4001)
         sel = [(int(j), int(c)) for j, c in zip(crng.choice(n, 32, replace=False), crng.integers(0, 3, 32))]
         sel = sel[:24] + [(j, 3) for j, _ in sel[24:]]
@@ -490,6 +491,8 @@ def bench_adjoint(args):
         fref = oracle.field_c(geom, at, nthreads=nt)
         rpos, rph, _ = oracle.adjoint_c(geom, at, 2 * (np.abs(fref) ** 2 - T), field=fref, nthreads=nt)
         check = {"fd_coordinates": len(sel), "fd_seconds": fd_s,
+                 "loss_rel_err_vs_fp64": abs(loss - float(np.sum((np.abs(fref) ** 2 - T) ** 2))) /
+                 float(np.sum((np.abs(fref) ** 2 - T) ** 2)),
                  "max_rel_err_vs_fd": float(np.max(np.abs(gpu - fd) / np.abs(fd))),
                  "max_err_vs_fd_over_abs_plus_rms": float(np.max(np.abs(gpu - fd) / (np.abs(fd) + rms))),
                  "rel_l2_vs_fp64_analytic": {"pos": rel_l2(gpos, rpos), "phase": rel_l2(gph, rph)}}
@@ -500,7 +503,7 @@ def bench_adjoint(args):
             "config": {"workload": "4096 emitters in a 20 nm ball perturbed by N(0, 0.05 nm), loss "
                                    "sum (I - I_target)^2, 256x256 px of 75 um at z=0.1 m, near field; "
                                    "step = forward + adjoint (pairs counted twice)", "pairs_per_step": pairs,
-                       "plan": h.plan()},
+                       "forward_plan": fwd_plan, "plan": h.plan()},
             "gradient_check": check}
 
 

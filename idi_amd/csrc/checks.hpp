@@ -87,12 +87,13 @@ inline int check_batched(const void* h, const double* emitters, const int64_t* o
     return IDISIM_OK;
 }
 
-inline int check_adjoint(const void* h, const double* emitters, int64_t n, const void* field, const float* dLdI,
-                         const float* grad_pos, const float* grad_phase, double extent, std::string* msg)
+// idisim_adjoint (G = dL/dI) and idisim_adjoint_l2 (G = the target intensity)
+inline int check_adjoint(const char* who, const void* h, const double* emitters, int64_t n, const void* field,
+                         const float* G, const float* grad_pos, const float* grad_phase, double extent,
+                         std::string* msg)
 {
-    const char* who = "idisim_adjoint";
     if (!h) return arg_error(msg, who, "NULL handle");
-    if (n < 0 || (n > 0 && (!emitters || !grad_pos || !grad_phase)) || !field || !dLdI)
+    if (n < 0 || (n > 0 && (!emitters || !grad_pos || !grad_phase)) || !field || !G)
         return arg_error(msg, who, "bad arguments");
     if (!std::isfinite(extent)) return arg_error(msg, who, "extent must be finite");
     return IDISIM_OK;

@@ -1077,19 +1077,27 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     return IDISIM_OK;
 }
 
-int idisim_adjoint(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
-                   const void* field, const float* dLdI, float* grad_pos, float* grad_phase, float* grad_amp,
-                   void* stream)
+// idisim_adjoint (target NULL) and idisim_adjoint_l2
+static int adjoint_call(const char* who, idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n,
+                        double extent, const void* field, const float* dLdI, const float* target, double* loss,
+                        float* grad_pos, float* grad_phase, float* grad_amp, void* stream)
 {
     g_err.clear();
     std::string msg;
-    if (int rc = check_adjoint(h, emitters, n, field, dLdI, grad_pos, grad_phase, extent, &msg)) return fail(rc, msg);
+    if (int rc = check_adjoint(who, h, emitters, n, field, target ? target : dLdI, grad_pos, grad_phase, extent, &msg))
+        return fail(rc, msg);
     if (h->precision != IDISIM_PREC_FP32SPLIT)
-        return fail(IDISIM_E_UNSUPPORTED, "idisim_adjoint: only FP32-split handles have an adjoint");
-    if (n == 0) return IDISIM_OK;
+        return fail(IDISIM_E_UNSUPPORTED, std::string(who) + ": only FP32-split handles have an adjoint");
     DeviceGuard guard(h->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (int rc0 = check_extent_flag(h, "idisim_adjoint")) return rc0;
+    if (n == 0) {
+        if (target != nullptr && loss != nullptr) {
+            // no emitters: E = 0 and the loss is sum T^2; no gradients to write
+            return fail(IDISIM_E_INVALID, std::string(who) + ": the loss needs at least one emitter");
+        }
+        return IDISIM_OK;
+    }
+    if (int rc0 = check_extent_flag(h, who)) return rc0;
     double rho = extent;
     if (!(rho > 0.0)) {
         int rc = measure_extent(h, emitters, n, st, &rho);
@@ -1099,7 +1107,7 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
     // |psi| <= 1/2) to the reduced phase: one more rounding at |tau + psi| <= 1 rev
     const double bound = plan_bound(h, 0, rho) + 0x1p-23;
     if (bound > h->tol_rev)
-        return fail(IDISIM_E_PRECISION, "idisim_adjoint: FP32-split error bound " + std::to_string(bound) +
+        return fail(IDISIM_E_PRECISION, std::string(who) + ": FP32-split error bound " + std::to_string(bound) +
                                             " rev exceeds the tolerance for this geometry");
     h->rho = rho;
     bool w = false;
@@ -1108,8 +1116,8 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
     const AdjointLayout L = adjoint_layout(h->g, n, h->num_cus, adjoint_occupancy(w, lam, grad_amp != nullptr));
     int rc = ensure_scratch(h, L.bytes);
     if (rc) return rc;
-    IDISIM_HIP(launch_adjoint(w, lam, gl, L, emitters, amplitudes, n, static_cast<const float2*>(field),
-                              dLdI, h->scratch, grad_pos, grad_phase, grad_amp, st));
+    IDISIM_HIP(launch_adjoint(w, lam, gl, L, emitters, amplitudes, n, static_cast<const float2*>(field), dLdI,
+                              target, loss, h->scratch, grad_pos, grad_phase, grad_amp, st));
     const int splits = L.splits;
     h->plan.precision = IDISIM_PREC_FP32SPLIT;
     h->plan.tile = 0;
@@ -1120,6 +1128,22 @@ int idisim_adjoint(idisim_handle* h, const double* emitters, const double* ampli
     h->plan.fallback = 0;
     h->plan.graph = 0;
     return IDISIM_OK;
+}
+
+int idisim_adjoint(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
+                   const void* field, const float* dLdI, float* grad_pos, float* grad_phase, float* grad_amp,
+                   void* stream)
+{
+    return adjoint_call("idisim_adjoint", h, emitters, amplitudes, n, extent, field, dLdI, nullptr, nullptr, grad_pos,
+                        grad_phase, grad_amp, stream);
+}
+
+int idisim_adjoint_l2(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
+                      const void* field, const float* target, double* loss, float* grad_pos, float* grad_phase,
+                      float* grad_amp, void* stream)
+{
+    return adjoint_call("idisim_adjoint_l2", h, emitters, amplitudes, n, extent, field, nullptr, target, loss,
+                        grad_pos, grad_phase, grad_amp, stream);
 }
 
 }  // extern "C"

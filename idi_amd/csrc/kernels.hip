@@ -230,6 +230,10 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
 // is folded into E_k instead).  The host planner bounds t^2 / 2 (REF 1) or |t|^3 / 6
 // (REF 2).  sD holds the four waves' D_k (FP32, 24 floats) and sE their phasors,
 // (cos, sin) of both pairs per float4, with REF 2 a second float4 scaled by sqrt 2.
+// Emitters staged in LDS per round by the mirror-pair tiles (records and, with a wave reference,
+// the four waves' phasors: 16 B x (1 + 4 or 8) per emitter).
+constexpr int kPairChunk = 256;
+
 template <int REF>
 __device__ __forceinline__ void pair_lane(int tid, int& lx, int& ly)
 {
@@ -347,10 +351,10 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             }
         }
     }
-    const float4* eref = sE + ((REF == 1 ? 1 : 2) * (tid >> 6)) * kChunk;  // this wave's phasors
+    const float4* eref = sE + ((REF == 1 ? 1 : 2) * (tid >> 6)) * kPairChunk;  // this wave's phasors
 
-    for (int64_t base = e0; base < e1; base += kChunk) {
-        const int cnt = (e1 - base) < (int64_t)kChunk ? (int)(e1 - base) : kChunk;
+    for (int64_t base = e0; base < e1; base += kPairChunk) {
+        const int cnt = (e1 - base) < (int64_t)kPairChunk ? (int)(e1 - base) : kPairChunk;
         __syncthreads();  // the previous chunk has been consumed by every lane
         if (tid < cnt) {
             const int64_t e = base + tid;
@@ -381,8 +385,8 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                             v[4 + 2 * k + s2] = (float)kTwoPi * gl;
                         }
                     }
-                    sE[(2 * w) * kChunk + tid] = make_float4(v[0], v[1], v[2], v[3]);
-                    sE[(2 * w + 1) * kChunk + tid] = make_float4(v[4], v[5], v[6], v[7]);
+                    sE[(2 * w) * kPairChunk + tid] = make_float4(v[0], v[1], v[2], v[3]);
+                    sE[(2 * w + 1) * kPairChunk + tid] = make_float4(v[4], v[5], v[6], v[7]);
                 }
             } else if (REF != 0) {
 #pragma unroll
@@ -398,10 +402,10 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                         es[k] = HAS_AMP ? r.amp * __builtin_amdgcn_sinf(dl) : __builtin_amdgcn_sinf(dl);
                         ec[k] = HAS_AMP ? r.amp * __builtin_amdgcn_cosf(dl) : __builtin_amdgcn_cosf(dl);
                     }
-                    sE[(REF * w) * kChunk + tid] = make_float4(ec[0], es[0], ec[1], es[1]);
+                    sE[(REF * w) * kPairChunk + tid] = make_float4(ec[0], es[0], ec[1], es[1]);
                     if (REF == 2) {
                         constexpr float r2 = 1.41421356237309504880f;
-                        sE[(REF * w + 1) * kChunk + tid] =
+                        sE[(REF * w + 1) * kPairChunk + tid] =
                             make_float4(r2 * ec[0], r2 * es[0], r2 * ec[1], r2 * es[1]);
                     }
                 }
@@ -422,7 +426,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             const float am = (HAS_AMP && REF == 0) ? s1[e].y : 1.f;
             float4 E = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E;
             if (REF != 0) E = eref[e];
-            if (REF >= 2) E2 = eref[kChunk + e];  // REF 2: sqrt 2 E; REF 3: the slopes
+            if (REF >= 2) E2 = eref[kPairChunk + e];  // REF 2: sqrt 2 E; REF 3: the slopes
 #pragma unroll
             for (int q = 0; q < QY; ++q) {
                 float t = fmaf(mc[q][0], r0.x, r0.w);
@@ -647,10 +651,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
                                                      int kind)
 {
     constexpr int TX = 32, TY = 32 * QY;
-    __shared__ float4 s0[kChunk];
-    __shared__ float2 s1[kChunk];
+    __shared__ float4 s0[kPairChunk];
+    __shared__ float2 s1[kPairChunk];
     __shared__ float sD[REF ? 72 : 1];
-    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kChunk : 1];
+    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kPairChunk : 1];
     const int tile = blockIdx.x;
     const int split = blockIdx.y;
     const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
@@ -775,10 +779,10 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* _
                                                          int* __restrict__ counter, int tiles_y, void* out, int kind)
 {
     constexpr int TX = 32, TY = 32 * QY;
-    __shared__ float4 s0[kChunk];
-    __shared__ float2 s1[kChunk];
+    __shared__ float4 s0[kPairChunk];
+    __shared__ float2 s1[kPairChunk];
     __shared__ float sD[REF ? 72 : 1];
-    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kChunk : 1];
+    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kPairChunk : 1];
     __shared__ int s_item;
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);

@@ -42,14 +42,17 @@ int occupancy_batched_fp64(int v);
 struct AdjointLayout {
     int tiles_y, ntiles, per, splits;
     int64_t egroups;
-    size_t pix_bytes, bytes;
+    size_t pix_bytes, part_bytes, bytes;  // pixel tables, split partials, all (with ntiles loss parts)
 };
 AdjointLayout adjoint_layout(const Geom& g, int64_t n, int num_cus, int blocks_per_cu);
 int adjoint_occupancy(bool w, bool lam, bool amp);  // resident workgroups per CU
 // w: lam_p W_cj enters the phase (not folded); lam: far-field secondorder with w.
+// target == nullptr: dL/dI is dLdI; else dL/dI = 2 (|E|^2 - target), and loss (device, may be
+// NULL) receives sum (|E|^2 - target)^2.
 hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& L, const double* emit,
-                          const double* amp, int64_t n, const float2* field, const float* dLdI, void* scratch,
-                          float* grad_pos, float* grad_phase, float* grad_amp, hipStream_t st);
+                          const double* amp, int64_t n, const float2* field, const float* dLdI, const float* target,
+                          double* loss, void* scratch, float* grad_pos, float* grad_phase, float* grad_amp,
+                          hipStream_t st);
 
 hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out, hipStream_t st);
 // issue-rate probe (kind 0: v_fma_f32, 1: v_sin_f32 / v_cos_f32); out: blocks x 256 floats

@@ -205,6 +205,17 @@ IDISIM_API int idisim_adjoint(idisim_handle *h, const double *emitters, const do
                    int64_t n, double extent, const void *field, const float *dLdI,
                    float *grad_pos, float *grad_phase, float *grad_amp, void *stream);
 
+/* idisim_adjoint for the squared-intensity loss L = sum_p (|E_p|^2 - T_p)^2 (no reference
+   counterpart; the loss of the fitting workload): dL/dI = 2 (|E|^2 - T) is formed from `field`
+   and `target` inside the adjoint's pixel pass, so the caller runs no kernels of its own.
+     target: float32 n0 x n1
+     loss:   one float64, DEVICE, or NULL: receives L (summed per 16 x 16 tile, then over the
+             tiles in a fixed order, so it is the same bit for bit on every call)
+   The other arguments are those of idisim_adjoint; n >= 1 when loss is requested. */
+IDISIM_API int idisim_adjoint_l2(idisim_handle *h, const double *emitters, const double *amplitudes,
+                                 int64_t n, double extent, const void *field, const float *target, double *loss,
+                                 float *grad_pos, float *grad_phase, float *grad_amp, void *stream);
+
 /* Largest |r_j| of n emitters (device reduction; synchronises `stream`). */
 IDISIM_API int idisim_extent(const double *emitters, int64_t n, double *extent_out, void *stream);
 

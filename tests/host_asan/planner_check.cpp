@@ -76,9 +76,10 @@ static int checks()
     add("batched_decreasing", check_batched(&h, buf, offs_dec, 2, buf, IDISIM_OUT_FIELD, 0.1, &m));
     add("batched_no_shots", check_batched(&h, buf, offs_ok, 0, buf, IDISIM_OUT_FIELD, 0.1, &m));
     add("batched_null_emitters", check_batched(&h, nullptr, offs_ok, 3, buf, IDISIM_OUT_FIELD, 0.1, &m));
-    add("adjoint_ok", check_adjoint(&h, buf, 4, buf, fbuf, fbuf, fbuf, 0.0, &m));
-    add("adjoint_null_grad", check_adjoint(&h, buf, 4, buf, fbuf, nullptr, fbuf, 0.0, &m));
-    add("adjoint_null_field", check_adjoint(&h, buf, 4, nullptr, fbuf, fbuf, fbuf, 0.0, &m));
+    add("adjoint_ok", check_adjoint("a", &h, buf, 4, buf, fbuf, fbuf, fbuf, 0.0, &m));
+    add("adjoint_null_grad", check_adjoint("a", &h, buf, 4, buf, fbuf, nullptr, fbuf, 0.0, &m));
+    add("adjoint_null_field", check_adjoint("a", &h, buf, 4, nullptr, fbuf, fbuf, fbuf, 0.0, &m));
+    add("adjoint_null_target", check_adjoint("a", &h, buf, 4, buf, nullptr, fbuf, fbuf, 0.0, &m));
     add("time_ok", check_time_forward(&h, buf, buf, 4, 0.3, 1.0, buf, &m));
     add("time_tau", check_time_forward(&h, buf, buf, 4, 0.3, 0.0, buf, &m));
     add("option_tile_ok", check_option(IDISIM_OPT_TILE, 13, kNumTiles, &m));

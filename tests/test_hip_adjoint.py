@@ -119,3 +119,67 @@ def test_autograd_function(lib, oracle_mod):
     assert rel_l2(pos.grad.cpu().numpy(), rpos) <= GRAD_RTOL
     assert rel_l2(ph.grad.cpu().numpy(), rph) <= GRAD_RTOL
     assert rel_l2(amp.grad.cpu().numpy(), ramp) <= GRAD_RTOL
+
+
+@pytest.mark.parametrize("mode", ["nf", "ff_secondorder", "nf_scale"])
+def test_adjoint_l2_fused_loss(lib, oracle_mod, mode):
+    """idisim_adjoint_l2 (dL/dI formed inside the adjoint) against idisim_adjoint with dL/dI from
+    torch, the oracle's FP64 loss and gradient; the loss is the same bit for bit on every call."""
+    import torch
+
+    from oracle import Geometry
+
+    rng = np.random.default_rng(93)
+    g = Geometry(64, 48, 300.0, 1e5, K_UM, nf=mode.startswith("nf"), secondorder="secondorder" in mode,
+                 scale="scale" in mode)
+    at, target = _problem(rng, 600, 0.02, g, oracle_mod)
+    amp = rng.uniform(0.5, 1, 600)
+    h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+    d, da = torch.from_numpy(at).cuda(), torch.from_numpy(amp).cuda()
+    T = torch.from_numpy(target).float().cuda()
+    field = h.forward(d, da)
+    loss, gpos, gph, gamp = h.adjoint_l2(d, field, T, da)
+    loss2, gpos2, gph2, gamp2 = h.adjoint_l2(d, field, T, da)
+    torch.cuda.synchronize()
+    assert loss.item() == loss2.item() and torch.equal(gpos, gpos2) and torch.equal(gph, gph2)
+    I = field.real.double() ** 2 + field.imag.double() ** 2
+    assert loss.item() == pytest.approx(float(((I - T.double()) ** 2).sum()), rel=1e-12)
+    spos, sph, samp = h.adjoint(d, field, (2 * (I - T.double())).float().contiguous(), da)
+    for a_, b_ in ((gpos, spos), (gph, sph), (gamp, samp)):
+        assert rel_l2(a_.cpu().numpy(), b_.cpu().numpy()) <= 1e-6
+    fref = oracle_mod.field_c(g, at, amp)
+    lref = np.sum((np.abs(fref) ** 2 - target.astype(np.float32).astype(np.float64)) ** 2)
+    assert loss.item() == pytest.approx(lref, rel=1e-4)
+    rpos, rph, ramp = oracle_mod.adjoint_c(g, at, 2 * (np.abs(fref) ** 2 - target.astype(np.float32)), field=fref,
+                                           amp=amp)
+    e = (rel_l2(gpos.cpu().numpy(), rpos), rel_l2(gph.cpu().numpy(), rph), rel_l2(gamp.cpu().numpy(), ramp))
+    assert max(e) <= GRAD_RTOL, e
+
+
+def test_autograd_l2_loss(lib, oracle_mod):
+    import torch
+
+    from idi_amd.simulation.autograd import intensity, intensity_l2_loss
+    from oracle import Geometry
+
+    rng = np.random.default_rng(94)
+    g = Geometry(32, 32, 150.0, 1e5, K_UM, nf=True)
+    at, target = _problem(rng, 300, 0.02, g, oracle_mod)
+    T = torch.from_numpy(target).float().cuda()
+    kw = dict(Ndet=32, pixelsize=150.0, detz=1e5, k=K_UM, settings="single nf")
+    grads = []
+    for fused in (True, False):
+        pos = torch.from_numpy(at[:, :3].copy()).cuda().requires_grad_(True)
+        ph = torch.from_numpy(at[:, 3].copy()).cuda().requires_grad_(True)
+        amp = torch.ones(300, dtype=torch.float64, device="cuda", requires_grad=True)
+        if fused:
+            loss = intensity_l2_loss(pos, ph, amp, target=T, **kw)
+        else:
+            I = intensity(pos, ph, amp, **kw)
+            loss = ((I.double() - T.double()) ** 2).sum()
+        (3.0 * loss).backward()  # the upstream gradient scales the stored ones
+        grads.append((loss.item(), pos.grad.cpu().numpy(), ph.grad.cpu().numpy(), amp.grad.cpu().numpy()))
+    (l1, *g1), (l2, *g2) = grads
+    assert l1 == pytest.approx(l2, rel=1e-6)
+    for a_, b_ in zip(g1, g2):
+        assert rel_l2(a_, b_) <= 1e-5
