@@ -301,6 +301,8 @@ def bench_batched(args):
     ext = 0.03
     d = torch.empty((int(offsets[-1]), 4), dtype=torch.float64, device="cuda")
     h = _lib.Handle(n0, n1, 100.0, 1e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    if args.tile is not None:  # pin a tile variant (profiling one kernel)
+        h.set_option(_lib.OPT_TILE, args.tile)
     out = torch.empty((nshots, n0, n1), dtype=torch.float32, device="cuda")
     seeds = iter(range(1, 1 << 30))
 
@@ -391,6 +393,8 @@ def bench_large(args):
     amp = torch.from_numpy(rng.uniform(0.5, 1.0, nshots * n)).cuda()
     ext = float(np.linalg.norm(pos, axis=1).max())
     h = _lib.Handle(n0, n1, 50.0, 5e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    if args.tile is not None:
+        h.set_option(_lib.OPT_TILE, args.tile)
     out = torch.empty((nshots, n0, n1), dtype=torch.float32, device="cuda")
     seeds = iter(range(1, 1 << 30))
 
@@ -851,7 +855,8 @@ def main():
                                                          "correlate", "generate", "issue"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile", type=int, default=None,
-                    help="single-image workload: run this FP32-split tile variant instead of the planner's choice")
+                    help="single, batched and large workloads: run this FP32-split tile variant instead of the "
+                         "planner's choice")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

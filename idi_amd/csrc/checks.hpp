@@ -37,6 +37,8 @@ inline Geom make_geom(const idisim_geometry& geom)
     g.nf = (geom.flags & IDISIM_NF) ? 1 : 0;
     g.scale = (geom.flags & IDISIM_SCALE) ? 1 : 0;
     g.secondorder = (geom.flags & IDISIM_SECONDORDER) ? 1 : 0;
+    g.refrho = 0.f;
+    g.refc = 0.f;
     return g;
 }
 

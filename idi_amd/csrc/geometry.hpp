@@ -20,7 +20,7 @@
 //   r'_cj = r_j (1 + sigma_cj),  sigma_cj = (s_cj - w_cj) / D_cj  (nf), 0 (ff)
 // The nf lines are the first-order expansion of w in (s, R) about the tile
 // centre (dw/ds = (w - s)/D, dw/dR = -w/D); the host planner bounds the
-// remainder (plan_bound in idisim.hip).  Far field (with or without
+// remainder (plan_bound in planner.hpp).  Far field (with or without
 // secondorder) is exact up to rounding.  A is formed in FP64 and rounded; W,
 // r', dq, lam are FP32; |dq . r'| is a few revolutions, so the FP32 phase
 // keeps ~1e-7 rev.
@@ -48,6 +48,11 @@ struct Geom {
     // memory that the host reads on its next call.  viol == nullptr: no check.
     double ext2;
     unsigned int* viol;
+    // Set per launch for tile variant 14: refrho = rho (1 + sigma), the largest |r'_cj|, and
+    // refc = T^2 / 4, T the planner's bound of the rotation angle t = (2 pi (d_k - D_k)) . r'.
+    // A lane whose own bound T_k = |2 pi (d_k - D_k)| refrho exceeds T / sqrt 2 scales its pair
+    // sums by 1 / (1 + T_k^2 / 2 - refc), which keeps the first-order truncation within T^2 / 4.
+    float refrho, refc;
 };
 
 struct Frame {
@@ -220,7 +225,12 @@ struct WorkItem {
 // 13: as 9, with the rotation angle linear in the quad's offset from the block
 // centre (two per-emitter slopes per pair and wave), so that a lane forms it in two
 // operations instead of three.
-constexpr int kNumTiles = 14;
+// 14: as 11 (centred pairs, first-order rotation), with a reference for each 8 x 8
+// pixel sub-block (four per wave, 16 per workgroup) instead of each wave block: the
+// rotation angle is about 3/7 as large, and the lanes with the largest angles scale
+// their sums so that the first-order truncation is at most T^2 / 4 instead of T^2 / 2;
+// the first order then qualifies where 11 needs the second order of 12.
+constexpr int kNumTiles = 15;
 constexpr int kTileNone = 4;  // reserved index, no kernel
 constexpr int kTilePair1 = 5;
 constexpr int kTilePair2 = 6;
@@ -231,20 +241,25 @@ constexpr int kTileRef2 = 10;
 constexpr int kTileRefC1 = 11;
 constexpr int kTileRefC2 = 12;
 constexpr int kTileRefL = 13;
-__host__ __device__ constexpr bool tile_pair(int v) { return v >= kTilePair1 && v <= kTileRefL; }
+constexpr int kTileRefS = 14;
+__host__ __device__ constexpr bool tile_pair(int v) { return v >= kTilePair1 && v <= kTileRefS; }
 __host__ __device__ constexpr bool tile_pair_centred(int v)
 {
-    return v == kTilePairC || v == kTileRefC1 || v == kTileRefC2;
+    return v == kTilePairC || v == kTileRefC1 || v == kTileRefC2 || v == kTileRefS;
 }
 // kind of wave reference: 0 none (own sincos), 1 first-order rotation, 2 second order,
-// 3 first order with the angle linear in the quad offset
+// 3 first order with the angle linear in the quad offset, 4 first order from a reference
+// per 8 x 8 pixel sub-block with magnitude-centred phasors
 __host__ __device__ constexpr int tile_ref_kind(int v)
 {
     return (v == kTileRef1 || v == kTileRefC1) ? 1
-           : ((v == kTileRef2 || v == kTileRefC2) ? 2 : (v == kTileRefL ? 3 : 0));
+           : ((v == kTileRef2 || v == kTileRefC2) ? 2 : (v == kTileRefL ? 3 : (v == kTileRefS ? 4 : 0)));
 }
-// order of the rotation from the wave reference to the lane's delta_k (0: own sincos)
-__host__ __device__ constexpr int tile_ref_order(int v) { return tile_ref_kind(v) == 3 ? 1 : tile_ref_kind(v); }
+// order of the rotation from the reference to the lane's delta_k (0: own sincos)
+__host__ __device__ constexpr int tile_ref_order(int v)
+{
+    return tile_ref_kind(v) >= 3 ? 1 : tile_ref_kind(v);
+}
 __host__ __device__ constexpr int pair_quads(int v) { return v == kTilePair2 ? 2 : 1; }
 __host__ __device__ constexpr int tile_px(int v) { return v >= 2 && v != kTileNone ? 2 : 1; }
 __host__ __device__ constexpr int tile_py(int v)

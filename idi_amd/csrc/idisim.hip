@@ -208,6 +208,10 @@ Geom split_geom(const idisim_handle* h, int tile, bool* w)
     const double e = h->rho * (1.0 + 1e-9);
     gl.ext2 = e * e;
     gl.viol = h->d_viol;
+    if (tile_ref_kind(tile) == 4) {
+        gl.refrho = (float)(h->rho * (1.0 + sigma_bound(h, tile, h->rho)));
+        gl.refc = (float)ref_trunc_centre(h, tile, h->rho);
+    }
     return gl;
 }
 
@@ -293,9 +297,11 @@ std::vector<int> split_candidates(const idisim_handle* h, int64_t n, int tile)
     const int tiles = ((h->g.n0 + tile_rows(tile) - 1) / tile_rows(tile)) *
                       ((h->g.n1 + tile_cols(tile) - 1) / tile_cols(tile));
     std::vector<int> out;
-    for (int s = 1; s <= 64 && s <= maxs; s *= 2) {
+    // powers of two, and 5 x 2^k for the kernels that hold five workgroups per CU (whole rounds)
+    for (int s : {1, 2, 4, 5, 8, 10, 16, 20, 32, 40, 64, 80}) {
+        if (s > maxs) break;
         out.push_back(s);
-        if ((int64_t)tiles * s >= 16LL * h->num_cus) break;  // 16 workgroups per CU is plenty
+        if ((int64_t)tiles * s >= 20LL * h->num_cus) break;  // 20 workgroups per CU is plenty
     }
     return out;
 }
@@ -365,7 +371,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     } else {
         // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
+        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRefS, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -1057,7 +1063,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
+        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRefS, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;

@@ -154,8 +154,27 @@ __global__ __launch_bounds__(kBlock) void k_fwd_split(Geom g, const double* __re
 // Items are sorted by emitter count (largest first) on the host, so the tail
 // of the queue is made of short items.  Every workgroup leaves the loop once
 // the counter passes nitems.
+// The Geom argument of a persistent kernel, read again from the kernarg segment for each work item
+// (a kernel's first argument sits at offset 0 of that segment).  Held as a by-value argument, all
+// of it stays live in SGPRs across the emitter loop of every item, and the excess is spilled into
+// VGPR lanes (32 SGPRs and 28 VGPRs in the mirror-pair kernels, one wave per SIMD fewer).  The
+// empty asm makes the pointer opaque, so the loads are not hoisted out of the item loop; they
+// are scalar loads from constant memory.
+__device__ __forceinline__ Geom item_geom()
+{
+    static_assert(sizeof(Geom) % 4 == 0, "Geom is copied as 32-bit words");
+    using KernargWords = const __attribute__((address_space(4))) uint32_t*;
+    KernargWords p = (KernargWords)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    Geom g;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&g);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(Geom) / 4); ++i) d[i] = p[i];
+    return g;
+}
+
 template <int PX, int PY, bool HAS_W, bool HAS_AMP, int SUM>
-__global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* __restrict__ emit,
+__global__ __launch_bounds__(kBlock) void k_batched_split(Geom g0, const double* __restrict__ emit,
                                                           const double* __restrict__ amp,
                                                           const WorkItem* __restrict__ items, int nitems,
                                                           int* __restrict__ counter, int tiles_y, void* out,
@@ -167,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
     __shared__ float2 s1[kChunk];
     __shared__ int s_item;
     const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
-    const size_t npix = (size_t)g.n0 * g.n1;
+    const size_t npix = (size_t)g0.n0 * g0.n1;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -175,21 +194,23 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
         const int it = s_item;
         if (it >= nitems) break;
         const WorkItem w = items[it];
+        const Geom g = item_geom();
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         float re[P], im[P], cre[P], cim[P];
         split_tile<PX, PY, HAS_W, HAS_AMP, SUM>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, re, im, cre,
                                                 cim);
         char* outs = static_cast<char*>(out) +
                      (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
+        const Geom gs = item_geom();  // store frames formed here (see k_batched_pair)
 #pragma unroll
         for (int i = 0; i < PX; ++i) {
 #pragma unroll
             for (int j = 0; j < PY; ++j) {
                 const int p = i * PY + j;
                 const int x = tx0 + lx * PX + i, y = ty0 + ly * PY + j;
-                if (x < g.n0 && y < g.n1)
-                    store_split(g, x, y, (double)re[p] - (double)cre[p], (double)im[p] - (double)cim[p],
-                                outs, kind, (size_t)x * g.n1 + y);
+                if (x < gs.n0 && y < gs.n1)
+                    store_split(gs, x, y, (double)re[p] - (double)cre[p], (double)im[p] - (double)cim[p],
+                                outs, kind, (size_t)x * gs.n1 + y);
             }
         }
     }
@@ -233,6 +254,29 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g, const double* 
 // Emitters staged in LDS per round by the mirror-pair tiles (records and, with a wave reference,
 // the four waves' phasors: 16 B x (1 + 4 or 8) per emitter).
 constexpr int kPairChunk = 256;
+// REF 4 (variant 14) takes a reference for each 8 x 8 pixel sub-block of a wave (four per wave)
+// and rotates to first order; the lanes far from their reference scale their sums at the end
+// (g.refrho, g.refc).  Its 16 phasors per emitter are
+// stored emitter-major with a stride of 17 float4 (a lane's four sub-blocks read adjacent
+// float4s; the record stage's writes from consecutive lanes fall in distinct banks), 128
+// emitters per round, so that the LDS of a workgroup stays under 40 KB (4 per CU).
+constexpr int kSubRefs = 16;
+constexpr int kSubStride = 17;
+// REF 3 (variant 13) holds two float4 per emitter and wave: 224 emitters per round keep a
+// workgroup's LDS under 32 KB, so that five workgroups share a CU.
+constexpr int kRef3Chunk = 224;
+__host__ __device__ constexpr int pair_chunk(int ref)
+{
+    return ref == 4 ? 128 : (ref == 3 ? kRef3Chunk : kPairChunk);
+}
+__host__ __device__ constexpr int pair_sd_floats(int ref) { return ref == 4 ? 6 * kSubRefs : (ref ? 72 : 1); }
+__host__ __device__ constexpr int pair_se_float4(int ref)
+{
+    return ref == 0 ? 1 : (ref == 1 ? 4 * kPairChunk : (ref == 4 ? kSubStride * 128 : 8 * pair_chunk(ref)));
+}
+// (W, amplitude) of each staged emitter: read only by the tiles without a reference (the
+// others fold the curvature into A and the amplitude into their reference phasors)
+__host__ __device__ constexpr int pair_s1_float2(int ref) { return ref == 0 ? kPairChunk : 1; }
 
 template <int REF>
 __device__ __forceinline__ void pair_lane(int tid, int& lx, int& ly)
@@ -296,10 +340,20 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             }
         }
     }
-    if (REF == 1 || REF == 2) {
-        const int w = tid >> 6;
-        const double px = (double)(tx0 + 16 * (w >> 1)) + 7.5;
-        const double py = (double)(ty0 + 16 * QY * (w & 1)) + (8.0 * QY - 0.5);
+    int myref = 0;  // REF 4: the lane's sub-block, 4 w + (2 x half + y half of the wave block)
+    if (REF == 1 || REF == 2 || REF == 4) {
+        static_assert(REF != 4 || (QY == 1 && CENTRED), "sub-block references: one centred quad per lane");
+        const int w = tid >> 6, l = tid & 63;
+        const int sb = ((l >> 5) << 1) | ((l >> 2) & 1);
+        myref = 4 * w + sb;
+        // the reference quad's centre: the wave block's centre (REF 1, 2) or the sub-block's
+        const double px = REF == 4 ? (double)(tx0 + 16 * (w >> 1) + 8 * (sb >> 1)) + 3.5
+                                   : (double)(tx0 + 16 * (w >> 1)) + 7.5;
+        const double py = REF == 4 ? (double)(ty0 + 16 * (w & 1) + 8 * (sb & 1)) + 3.5
+                                   : (double)(ty0 + 16 * QY * (w & 1)) + (8.0 * QY - 0.5);
+        // the lane that stores its reference: the first of the wave (REF 1, 2) or of the sub-block
+        const bool writer = REF == 4 ? (l & 27) == 0 : l == 0;
+        float* Dst = REF == 4 ? sD + 6 * myref : sD + 18 * w;
         const Frame f00 = pixel_frame(g, px - 0.5, py - 0.5), f11 = pixel_frame(g, px + 0.5, py + 0.5);
         const Frame f10 = pixel_frame(g, px + 0.5, py - 0.5), f01 = pixel_frame(g, px - 0.5, py + 0.5);
 #pragma unroll
@@ -309,14 +363,14 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             const float D0 = (float)(0.5 * (q11 - q00)), D1 = (float)(0.5 * (q01 - q10));
             Dref[0][a] = (double)D0;
             Dref[1][a] = (double)D1;
-            if ((tid & 63) == 0) {
-                sD[w * 18 + a] = D0;
-                sD[w * 18 + 3 + a] = D1;
+            if (writer) {
+                Dst[a] = D0;
+                Dst[3 + a] = D1;
             }
         }
     }
-    // REF 1: t in radians; REF 2: t / sqrt 2, so that its square is t^2 / 2
-    const double dscale = REF == 1 ? kTwoPi : kTwoPi * 0.70710678118654752440;
+    // REF 1, 4: t in radians; REF 2: t / sqrt 2, so that its square is t^2 / 2
+    const double dscale = REF == 2 ? kTwoPi * 0.70710678118654752440 : kTwoPi;
 
     float mc[QY][3], d[QY][2][3], dv[QY][3];
 #pragma unroll
@@ -351,19 +405,40 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             }
         }
     }
-    const float4* eref = sE + ((REF == 1 ? 1 : 2) * (tid >> 6)) * kPairChunk;  // this wave's phasors
+    constexpr int CH = pair_chunk(REF);
+    // this lane's phasors: its wave's (REF 1..3, emitter stride 1) or its sub-block's (REF 4)
+    constexpr int ESTR = REF == 4 ? kSubStride : 1;
+    const float4* eref = REF == 4 ? sE + myref : sE + ((REF == 1 ? 1 : 2) * (tid >> 6)) * CH;
 
-    for (int64_t base = e0; base < e1; base += kPairChunk) {
-        const int cnt = (e1 - base) < (int64_t)kPairChunk ? (int)(e1 - base) : kPairChunk;
+    for (int64_t base = e0; base < e1; base += CH) {
+        const int cnt = (e1 - base) < (int64_t)CH ? (int)(e1 - base) : CH;
+        // REF 4: lanes tid and tid + 128 both form the record of emitter tid and each stores 8 of
+        // its 16 sub-block phasors
+        const int el = REF == 4 ? (tid & (CH - 1)) : tid;
         __syncthreads();  // the previous chunk has been consumed by every lane
-        if (tid < cnt) {
-            const int64_t e = base + tid;
+        if (el < cnt) {
+            const int64_t e = base + el;
             const double* a4 = emit + 4 * e;
             const double ea = (HAS_AMP && amp != nullptr) ? amp[e] : 1.0;
             const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], ea);
-            s0[tid] = make_float4(r.x, r.y, r.z, r.A);
-            s1[tid] = make_float2(r.W, r.amp);
-            if (REF == 3) {
+            if (REF != 4 || tid < CH) s0[el] = make_float4(r.x, r.y, r.z, r.A);
+            if (REF == 0) s1[el] = make_float2(r.W, r.amp);
+            if (REF == 4) {
+                const float sc = HAS_AMP ? r.amp : 1.f;
+                const int first = (tid >> 7) * (kSubRefs / 2);
+#pragma unroll 2
+                for (int j = 0; j < kSubRefs / 2; ++j) {
+                    const float* D = sD + 6 * (first + j);
+                    float dl0 = D[0] * r.x, dl1 = D[3] * r.x;
+                    dl0 = fmaf(D[1], r.y, dl0);
+                    dl1 = fmaf(D[4], r.y, dl1);
+                    dl0 = fmaf(D[2], r.z, dl0);
+                    dl1 = fmaf(D[5], r.z, dl1);
+                    sE[el * kSubStride + first + j] =
+                        make_float4(sc * __builtin_amdgcn_cosf(dl0), sc * __builtin_amdgcn_sinf(dl0),
+                                    sc * __builtin_amdgcn_cosf(dl1), sc * __builtin_amdgcn_sinf(dl1));
+                }
+            } else if (REF == 3) {
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
                     const float* D = sD + w * 18;
@@ -385,8 +460,8 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                             v[4 + 2 * k + s2] = (float)kTwoPi * gl;
                         }
                     }
-                    sE[(2 * w) * kPairChunk + tid] = make_float4(v[0], v[1], v[2], v[3]);
-                    sE[(2 * w + 1) * kPairChunk + tid] = make_float4(v[4], v[5], v[6], v[7]);
+                    sE[(2 * w) * CH + tid] = make_float4(v[0], v[1], v[2], v[3]);
+                    sE[(2 * w + 1) * CH + tid] = make_float4(v[4], v[5], v[6], v[7]);
                 }
             } else if (REF != 0) {
 #pragma unroll
@@ -402,10 +477,10 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                         es[k] = HAS_AMP ? r.amp * __builtin_amdgcn_sinf(dl) : __builtin_amdgcn_sinf(dl);
                         ec[k] = HAS_AMP ? r.amp * __builtin_amdgcn_cosf(dl) : __builtin_amdgcn_cosf(dl);
                     }
-                    sE[(REF * w) * kPairChunk + tid] = make_float4(ec[0], es[0], ec[1], es[1]);
+                    sE[(REF * w) * CH + tid] = make_float4(ec[0], es[0], ec[1], es[1]);
                     if (REF == 2) {
                         constexpr float r2 = 1.41421356237309504880f;
-                        sE[(REF * w + 1) * kPairChunk + tid] =
+                        sE[(REF * w + 1) * CH + tid] =
                             make_float4(r2 * ec[0], r2 * es[0], r2 * ec[1], r2 * es[1]);
                     }
                 }
@@ -425,8 +500,8 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             const float4 r0 = s0[e];
             const float am = (HAS_AMP && REF == 0) ? s1[e].y : 1.f;
             float4 E = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E;
-            if (REF != 0) E = eref[e];
-            if (REF >= 2) E2 = eref[kPairChunk + e];  // REF 2: sqrt 2 E; REF 3: the slopes
+            if (REF != 0) E = eref[e * ESTR];
+            if (REF == 2 || REF == 3) E2 = eref[CH + e];  // REF 2: sqrt 2 E; REF 3: the slopes
 #pragma unroll
             for (int q = 0; q < QY; ++q) {
                 float t = fmaf(mc[q][0], r0.x, r0.w);
@@ -462,7 +537,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                         cd = __builtin_amdgcn_cosf(dl);
                     } else {
                         const float ec = k ? E.z : E.x, es = k ? E.w : E.y;
-                        if (REF == 1 || REF == 3) {  // (ec + i es)(1 + i t), t = dl
+                        if (REF != 2) {  // (ec + i es)(1 + i t), t = dl
                             cd = fmaf(-dl, es, ec);
                             sd = fmaf(dl, ec, es);
                         } else {         // (ec + i es)(1 - u^2 + i sqrt2 u), u = dl
@@ -485,6 +560,24 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             for (int k = 0; k < 2; ++k)
 #pragma unroll
                 for (int m = 0; m < 4; ++m) kahan_add(acc[q][k][m], cmp[q][k][m], su[q][k][m]);
+    }
+    if (REF == 4) {
+        // the lane's own angle bound T_k = |2 pi (d_k - D_k)| max|r'|; beyond T / sqrt 2 its sums
+        // take the factor 1 / (1 + T_k^2 / 2 - T^2 / 4) (the planner's truncation bound)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const float t2 = (d[0][k][0] * d[0][k][0] + d[0][k][1] * d[0][k][1] + d[0][k][2] * d[0][k][2]) *
+                             (g.refrho * g.refrho);
+            const float cs = fmaxf(0.f, 0.5f * t2 - g.refc);
+            if (cs > 0.f) {
+                const float sc = 1.f / (1.f + cs);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    acc[0][k][m] *= sc;
+                    cmp[0][k][m] *= sc;
+                }
+            }
+        }
     }
 }
 
@@ -651,10 +744,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
                                                      int kind)
 {
     constexpr int TX = 32, TY = 32 * QY;
-    __shared__ float4 s0[kPairChunk];
-    __shared__ float2 s1[kPairChunk];
-    __shared__ float sD[REF ? 72 : 1];
-    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kPairChunk : 1];
+    __shared__ float4 s0[pair_chunk(REF)];
+    __shared__ float2 s1[pair_s1_float2(REF)];
+    __shared__ float sD[pair_sd_floats(REF)];
+    __shared__ float4 sE[pair_se_float4(REF)];
     const int tile = blockIdx.x;
     const int split = blockIdx.y;
     const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
@@ -665,7 +758,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
 
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);
-    const size_t npix = (size_t)g.n0 * g.n1;
+    const Geom gs = item_geom();  // store frames formed here (see k_batched_pair)
+    const size_t npix = (size_t)gs.n0 * gs.n1;
 #pragma unroll
     for (int q = 0; q < QY; ++q) {
 #pragma unroll
@@ -673,17 +767,17 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int x = tx0 + 2 * lx + i, y = ty0 + 2 * QY * ly + 2 * q + j;
-                if (x < g.n0 && y < g.n1) {
+                if (x < gs.n0 && y < gs.n1) {
                     double re, im;
                     pair_pixel<QY>(acc, cmp, q, i, j, re, im);
-                    const size_t idx = (size_t)x * g.n1 + y;
+                    const size_t idx = (size_t)x * gs.n1 + y;
                     if (partial != nullptr) {
                         float rh, rl, ih, il;
                         split_hi_lo(re, rh, rl);
                         split_hi_lo(im, ih, il);
                         partial[(size_t)split * npix + idx] = make_float4(rh, ih, rl, il);
                     } else {
-                        store_split(g, x, y, re, im, out, kind, idx);
+                        store_split(gs, x, y, re, im, out, kind, idx);
                     }
                 }
             }
@@ -733,7 +827,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_block(Geom g, const double* __re
 }
 
 template <bool HAS_AMP>
-__global__ __launch_bounds__(kBlock) void k_batched_block(Geom g, const double* __restrict__ emit,
+__global__ __launch_bounds__(kBlock) void k_batched_block(Geom g0, const double* __restrict__ emit,
                                                           const double* __restrict__ amp,
                                                           const WorkItem* __restrict__ items, int nitems,
                                                           int* __restrict__ counter, int tiles_y, void* out, int kind)
@@ -743,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_block(Geom g, const double* 
     __shared__ float2 s1[kChunk];
     __shared__ int s_item;
     const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
-    const size_t npix = (size_t)g.n0 * g.n1;
+    const size_t npix = (size_t)g0.n0 * g0.n1;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -751,20 +845,22 @@ __global__ __launch_bounds__(kBlock) void k_batched_block(Geom g, const double* 
         const int it = s_item;
         if (it >= nitems) break;
         const WorkItem w = items[it];
+        const Geom g = item_geom();
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         float acc[4][4], cmp[4][4];
         block_tile<HAS_AMP>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, acc, cmp);
         char* outs = static_cast<char*>(out) +
                      (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
+        const Geom gs = item_geom();  // store frames formed here (see k_batched_pair)
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int x = tx0 + 2 * lx + i, y = ty0 + 4 * ly + j;
-                if (x < g.n0 && y < g.n1) {
+                if (x < gs.n0 && y < gs.n1) {
                     double re, im;
                     block_pixel(acc, cmp, i, j, re, im);
-                    store_split(g, x, y, re, im, outs, kind, (size_t)x * g.n1 + y);
+                    store_split(gs, x, y, re, im, outs, kind, (size_t)x * gs.n1 + y);
                 }
             }
         }
@@ -773,20 +869,20 @@ __global__ __launch_bounds__(kBlock) void k_batched_block(Geom g, const double* 
 
 // Persistent batched form of k_fwd_pair (the queue protocol of k_batched_split).
 template <int QY, bool HAS_AMP, bool CENTRED, int REF>
-__global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* __restrict__ emit,
+__global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g0, const double* __restrict__ emit,
                                                          const double* __restrict__ amp,
                                                          const WorkItem* __restrict__ items, int nitems,
                                                          int* __restrict__ counter, int tiles_y, void* out, int kind)
 {
     constexpr int TX = 32, TY = 32 * QY;
-    __shared__ float4 s0[kPairChunk];
-    __shared__ float2 s1[kPairChunk];
-    __shared__ float sD[REF ? 72 : 1];
-    __shared__ float4 sE[REF ? (REF == 1 ? 4 : 8) * kPairChunk : 1];
+    __shared__ float4 s0[pair_chunk(REF)];
+    __shared__ float2 s1[pair_s1_float2(REF)];
+    __shared__ float sD[pair_sd_floats(REF)];
+    __shared__ float4 sE[pair_se_float4(REF)];
     __shared__ int s_item;
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);
-    const size_t npix = (size_t)g.n0 * g.n1;
+    const size_t npix = (size_t)g0.n0 * g0.n1;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -794,11 +890,15 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* _
         const int it = s_item;
         if (it >= nitems) break;
         const WorkItem w = items[it];
+        const Geom g = item_geom();
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         float acc[QY][2][4], cmp[QY][2][4];
         pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, sD, sE, acc, cmp);
         char* outs = static_cast<char*>(out) +
                      (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
+        // read again, so that the pixel frames of the stores are formed here and not kept in
+        // VGPRs from the prologue's identical frames across the emitter loop
+        const Geom gs = item_geom();
 #pragma unroll
         for (int q = 0; q < QY; ++q) {
 #pragma unroll
@@ -806,10 +906,10 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g, const double* _
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     const int x = tx0 + 2 * lx + i, y = ty0 + 2 * QY * ly + 2 * q + j;
-                    if (x < g.n0 && y < g.n1) {
+                    if (x < gs.n0 && y < gs.n1) {
                         double re, im;
                         pair_pixel<QY>(acc, cmp, q, i, j, re, im);
-                        store_split(g, x, y, re, im, outs, kind, (size_t)x * g.n1 + y);
+                        store_split(gs, x, y, re, im, outs, kind, (size_t)x * gs.n1 + y);
                     }
                 }
             }
@@ -960,7 +1060,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_fp64(Geom g, const double* __res
 }
 
 template <int PX, int PY>
-__global__ __launch_bounds__(kBlock) void k_batched_fp64(Geom g, const double* __restrict__ emit,
+__global__ __launch_bounds__(kBlock) void k_batched_fp64(Geom g0, const double* __restrict__ emit,
                                                          const double* __restrict__ amp,
                                                          const WorkItem* __restrict__ items, int nitems,
                                                          int* __restrict__ counter, int tiles_y, void* out,
@@ -972,7 +1072,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_fp64(Geom g, const double* _
     __shared__ double s1[kChunk];
     __shared__ int s_item;
     const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
-    const size_t npix = (size_t)g.n0 * g.n1;
+    const size_t npix = (size_t)g0.n0 * g0.n1;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -980,6 +1080,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_fp64(Geom g, const double* _
         const int it = s_item;
         if (it >= nitems) break;
         const WorkItem w = items[it];
+        const Geom g = item_geom();
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         double re[P], im[P], cre[P], cim[P];
         fp64_tile<PX, PY>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, re, im, cre, cim);
@@ -1190,23 +1291,25 @@ static int occupancy_pair_q()
 // (quads per lane, reference kind, centred, amplitudes) -> instantiation; only
 // variant 6 has two quads per lane
 #define IDISIM_DISPATCH_PAIR(FN, ...)                                                          \
-    switch ((pair_quads(v) << 4) | (tile_ref_kind(v) << 2) | ((int)tile_pair_centred(v) << 1) | (int)a) { \
-        case 16: return FN<1, false, false, 0>(__VA_ARGS__);                                  \
-        case 17: return FN<1, true, false, 0>(__VA_ARGS__);                                   \
-        case 18: return FN<1, false, true, 0>(__VA_ARGS__);                                   \
-        case 19: return FN<1, true, true, 0>(__VA_ARGS__);                                    \
-        case 20: return FN<1, false, false, 1>(__VA_ARGS__);                                  \
-        case 21: return FN<1, true, false, 1>(__VA_ARGS__);                                   \
-        case 22: return FN<1, false, true, 1>(__VA_ARGS__);                                   \
-        case 23: return FN<1, true, true, 1>(__VA_ARGS__);                                    \
-        case 24: return FN<1, false, false, 2>(__VA_ARGS__);                                  \
-        case 25: return FN<1, true, false, 2>(__VA_ARGS__);                                   \
-        case 26: return FN<1, false, true, 2>(__VA_ARGS__);                                   \
-        case 27: return FN<1, true, true, 2>(__VA_ARGS__);                                    \
-        case 28: return FN<1, false, false, 3>(__VA_ARGS__);                                  \
-        case 29: return FN<1, true, false, 3>(__VA_ARGS__);                                   \
-        case 32: return FN<2, false, false, 0>(__VA_ARGS__);                                  \
-        case 33: return FN<2, true, false, 0>(__VA_ARGS__);                                   \
+    switch ((pair_quads(v) << 5) | (tile_ref_kind(v) << 2) | ((int)tile_pair_centred(v) << 1) | (int)a) { \
+        case 32: return FN<1, false, false, 0>(__VA_ARGS__);                                  \
+        case 33: return FN<1, true, false, 0>(__VA_ARGS__);                                   \
+        case 34: return FN<1, false, true, 0>(__VA_ARGS__);                                   \
+        case 35: return FN<1, true, true, 0>(__VA_ARGS__);                                    \
+        case 36: return FN<1, false, false, 1>(__VA_ARGS__);                                  \
+        case 37: return FN<1, true, false, 1>(__VA_ARGS__);                                   \
+        case 38: return FN<1, false, true, 1>(__VA_ARGS__);                                   \
+        case 39: return FN<1, true, true, 1>(__VA_ARGS__);                                    \
+        case 40: return FN<1, false, false, 2>(__VA_ARGS__);                                  \
+        case 41: return FN<1, true, false, 2>(__VA_ARGS__);                                   \
+        case 42: return FN<1, false, true, 2>(__VA_ARGS__);                                   \
+        case 43: return FN<1, true, true, 2>(__VA_ARGS__);                                    \
+        case 44: return FN<1, false, false, 3>(__VA_ARGS__);                                  \
+        case 45: return FN<1, true, false, 3>(__VA_ARGS__);                                   \
+        case 50: return FN<1, false, true, 4>(__VA_ARGS__);                                   \
+        case 51: return FN<1, true, true, 4>(__VA_ARGS__);                                    \
+        case 64: return FN<2, false, false, 0>(__VA_ARGS__);                                  \
+        case 65: return FN<2, true, false, 0>(__VA_ARGS__);                                   \
         default: break;                                                                       \
     }
 

@@ -35,6 +35,7 @@ struct Planner {
     double pair_coef = 0.0;   // mirror-pair remainder per unit emitter extent (rev per length)
     double block_coef = 0.0;  // bound of |m_k - m_1| of the 2 x 4 block pairs per unit extent
     double ref_coef = 0.0;    // bound of |d_k - D_k| (lane pair vs wave reference) per unit extent
+    double refs_coef = 0.0;   // the same against the 8 x 8 pixel sub-block reference (variant 14)
     double lin_coef = 0.0;    // largest residual of the linear wave reference (variant 13) per unit extent
 };
 
@@ -129,6 +130,10 @@ inline void planner_tables(Planner* h)
         const double sr = std::min(1.0, std::max(ex, ey) / Rp);
         h->ref_coef = 1.05 * (1.0 + (2.0 + std::sqrt(2.0)) * sr + 6.0 * sr * sr) * g.krev * g.pix * g.pix *
                       (0.5 * std::sqrt(2.0)) * (7.0 * std::sqrt(2.0)) / (Rp * Rp);
+        // 8 x 8 pixel sub-blocks (variant 14): the quads lie at |o| <= 3 sqrt 2 pixels from the
+        // sub-block's reference quad, inside the same padded detector.  Replaced below by the
+        // exact table where the detector is small enough.
+        h->refs_coef = h->ref_coef * (3.0 / 7.0);
     }
     // Linear wave reference (variant 13): the kernel rotates by (ox Jx_k + oy Jy_k) . r' in place
     // of (d_k - D_k) . r'.  The residual d_k - D_k - ox Jx_k - oy Jy_k is evaluated here for every
@@ -136,9 +141,12 @@ inline void planner_tables(Planner* h)
     // wave_linear_ref), and its largest norm (with 5% margin for the FP64 rounding of this
     // evaluation) enters the bound.  Detectors beyond 2048 x 2048 pixels skip the table and the
     // variant.
+    // The sub-block references of variant 14 are tabulated in the same pass: the largest norm of
+    // d_k - D_k over every quad, D_k the FP32 half differences of the sub-block's reference quad
+    // (lower corner 3 pixels into the 8 x 8 sub-block) as the kernel forms them.
     h->lin_coef = std::numeric_limits<double>::infinity();
     if ((int64_t)g.n0 * g.n1 <= (int64_t)2048 * 2048) {
-        double worst = 0.0;
+        double worst = 0.0, worst_sub = 0.0;
         for (int tx0 = 0; tx0 < g.n0; tx0 += 32) {
             for (int ty0 = 0; ty0 < g.n1; ty0 += 32) {
                 const int xe = std::min(tx0 + 32, g.n0) - 1, ye = std::min(ty0 + 32, g.n1) - 1;
@@ -148,19 +156,30 @@ inline void planner_tables(Planner* h)
                     if (bx >= g.n0 || by >= g.n1) continue;
                     float D[2][3], Jx[2][3], Jy[2][3];
                     wave_linear_ref(g, c.u, bx + 7.5, by + 7.5, D, Jx, Jy);
+                    float Ds[4][2][3];
+                    for (int sb = 0; sb < 4; ++sb) {
+                        double hs[2][3];
+                        quad_half_diff(g, c.u, (double)(bx + 8 * (sb >> 1) + 3), (double)(by + 8 * (sb & 1) + 3), hs);
+                        for (int k = 0; k < 2; ++k)
+                            for (int a = 0; a < 3; ++a) Ds[sb][k][a] = (float)hs[k][a];
+                    }
                     for (int i = 0; i < 8 && bx + 2 * i < g.n0; ++i) {
                         for (int j = 0; j < 8 && by + 2 * j < g.n1; ++j) {
                             double hq[2][3];
                             quad_half_diff(g, c.u, (double)(bx + 2 * i), (double)(by + 2 * j), hq);
                             const double ox = 2 * i - 7, oy = 2 * j - 7;
+                            const int sb = ((i >> 2) << 1) | (j >> 2);
                             for (int k = 0; k < 2; ++k) {
-                                double n2 = 0.0;
+                                double n2 = 0.0, s2 = 0.0;
                                 for (int a = 0; a < 3; ++a) {
                                     const double r = hq[k][a] - (double)D[k][a] - ox * (double)Jx[k][a] -
                                                      oy * (double)Jy[k][a];
+                                    const double rs = hq[k][a] - (double)Ds[sb][k][a];
                                     n2 += r * r;
+                                    s2 += rs * rs;
                                 }
                                 worst = std::max(worst, std::sqrt(n2));
+                                worst_sub = std::max(worst_sub, std::sqrt(s2));
                             }
                         }
                     }
@@ -168,6 +187,7 @@ inline void planner_tables(Planner* h)
             }
         }
         h->lin_coef = 1.05 * worst;
+        h->refs_coef = 1.05 * worst_sub;
     }
 }
 
@@ -184,6 +204,40 @@ inline double fold_error(const Planner* h, int v, double rho)
     return W * dR / (h->Rmin - dR);
 }
 
+// Bound of sigma_cj = (s - w) / D (near field; 0 otherwise) for tile variant v and emitters within
+// |r_j| <= rho; infinite where the tile reaches the emitters.
+inline double sigma_bound(const Planner* h, int v, double rho)
+{
+    const double Rlo = h->Rmin - h->dR_max[v] - rho;
+    if (!(Rlo > 0.0)) return std::numeric_limits<double>::infinity();
+    if (!h->g.nf) return 0.0;
+    return (rho + rho * rho / (2.0 * Rlo)) / Rlo;
+}
+
+// Bound of the angle (rad) by which a lane rotates the reference phasor of variants 9..14 to its
+// own pair: t = 2 pi (d_k - D_k) . r' with |r'| <= rho (1 + sigma); for the linear reference (13)
+// the residual of its linear form is added.  0 for the tiles without a reference.
+inline double ref_angle_bound(const Planner* h, int v, double rho)
+{
+    const int kind = tile_ref_kind(v);
+    if (kind == 0) return 0.0;
+    const double coef = kind == 4 ? h->refs_coef : h->ref_coef + (kind == 3 ? h->lin_coef : 0.0);
+    return kTwoPi * coef * rho * (1.0 + sigma_bound(h, v, rho));
+}
+
+// Variant 14's truncation centre T^2 / 4, T the angle bound (0 for the other variants).  A lane
+// whose own bound T_k (|t| <= T_k <= T over its emitters) exceeds T / sqrt 2 scales its sums by
+// s = 1 / (1 + c), c = T_k^2 / 2 - T^2 / 4 in (0, T^2 / 4]; then s (1 + i t) - e^{i t} has real
+// part t^2 / 2 - c + O(T^4), within [-T^2 / 4, T^2 / 4], and imaginary part at most 5 T^3 / 12.
+// The other lanes keep s = 1 and their t^2 / 2 <= T^2 / 4.  The truncation is T^2 / 4 + T^3 / 2
+// instead of T^2 / 2, and only the lanes far from their reference carry the scale's bias.
+inline double ref_trunc_centre(const Planner* h, int v, double rho)
+{
+    if (tile_ref_kind(v) != 4) return 0.0;
+    const double T = ref_angle_bound(h, v, rho);
+    return std::isfinite(T) ? 0.25 * T * T : 0.0;
+}
+
 // A-priori bound of the per-term phase error (revolutions) of tile variant v
 // for emitters within |r_j| <= rho.
 inline double plan_bound(const Planner* h, int v, double rho)
@@ -194,13 +248,9 @@ inline double plan_bound(const Planner* h, int v, double rho)
     if (!(Rlo > 0.0)) return std::numeric_limits<double>::infinity();
     const double Rhi = h->Rmax + dR + rho;
     const double ds = du * rho;
-    double wmax = 0.0, sig = 0.0;
-    if (g.nf) {
-        wmax = rho * rho / (2.0 * Rlo);
-        sig = (rho + wmax) / Rlo;
-    } else if (g.secondorder) {
-        wmax = rho * rho / (2.0 * Rlo);
-    }
+    double wmax = 0.0;
+    const double sig = sigma_bound(h, v, rho);
+    if (g.nf || g.secondorder) wmax = rho * rho / (2.0 * Rlo);
     const double W = g.krev * wmax;
     const double B = g.krev * ds * (1.0 + sig);
     const double tau = 0.5 + W + B;
@@ -244,14 +294,18 @@ inline double plan_bound(const Planner* h, int v, double rho)
         e += kSinCosErrRev;  // second sincos (delta_k, or the reference E_k)
         e += 2.0 * u24;      // e^{i tau_c} (cos, sin) delta_k products, as phase
         if (const int ord = tile_ref_order(v)) {
-            // rotation from the wave reference, t = 2 pi (d_k - D_k) . r': truncation t^2 / 2
-            // (first order) or |t|^3 / 6 (second order), t's own FP32 rounding, and the
-            // rounding of D_k and of the rotated (cos, sin).  The linear reference (variant 13)
-            // rotates by the linear part of d_k - D_k instead: its residual enters as a phase
-            // error of its own, and its slopes are formed and combined in FP32 (8 roundings).
-            const bool lin = tile_ref_kind(v) == 3;
-            const double t = kTwoPi * (h->ref_coef + (lin ? h->lin_coef : 0.0)) * rho * (1.0 + sig);
-            const double trunc = ord == 1 ? 0.5 * t * t : t * t * t / 6.0;
+            // rotation from the reference, t = 2 pi (d_k - D_k) . r': truncation t^2 / 2 (first
+            // order), |t|^3 / 6 (second order) or T^2 / 4 + T^3 / 2 (first order with the sums of the
+            // far lanes scaled, variant 14, plus the rounding of that product), t's own FP32
+            // rounding, and the rounding of D_k and of the rotated (cos, sin).  The linear
+            // reference (variant 13) rotates by the linear part of d_k - D_k instead: its
+            // residual enters as a phase error of its own, and its slopes are formed and combined
+            // in FP32 (8 roundings).
+            const int kind = tile_ref_kind(v);
+            const bool lin = kind == 3;
+            const double t = ref_angle_bound(h, v, rho);
+            const double trunc = kind == 4 ? 0.25 * t * t + 0.5 * t * t * t + u24
+                                           : (ord == 1 ? 0.5 * t * t : t * t * t / 6.0);
             e += (trunc + (lin ? 8.0 : 4.0) * u24 * t) * kInv2Pi + 2.0 * u24 * B + 2.0 * u24;
             if (lin) e += h->lin_coef * rho * (1.0 + sig);
         }

@@ -79,9 +79,10 @@ extern "C" {
                                      with the pair phasors rotated from a
                                      per-wave reference to first (9, 11) or
                                      second (10, 12) order, 13 as 9 with the
-                                     rotation angle linear in the quad offset
-                                     (FP32-split only, 32x32 px); FP64 handles
-                                     take 0..3                                   */
+                                     rotation angle linear in the quad offset,
+                                     14 as 11 with a reference per 8x8 px
+                                     sub-block (FP32-split only, 32x32 px);
+                                     FP64 handles take 0..3                      */
 #define IDISIM_OPT_SPLIT        3 /* force emitter split count (0 = planner)        */
 #define IDISIM_OPT_SUMMODE      4 /* 0 (default): per-chunk partials merged with
                                      Kahan; 1: Kahan on every term (cpu.py:97-105) */

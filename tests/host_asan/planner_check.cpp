@@ -82,7 +82,7 @@ static int checks()
     add("adjoint_null_target", check_adjoint("a", &h, buf, 4, buf, nullptr, fbuf, fbuf, 0.0, &m));
     add("time_ok", check_time_forward(&h, buf, buf, 4, 0.3, 1.0, buf, &m));
     add("time_tau", check_time_forward(&h, buf, buf, 4, 0.3, 0.0, buf, &m));
-    add("option_tile_ok", check_option(IDISIM_OPT_TILE, 13, kNumTiles, &m));
+    add("option_tile_ok", check_option(IDISIM_OPT_TILE, kTileRefS, kNumTiles, &m));
     add("option_tile_reserved", check_option(IDISIM_OPT_TILE, kTileNone, kNumTiles, &m));
     add("option_tile_range", check_option(IDISIM_OPT_TILE, kNumTiles, kNumTiles, &m));
     add("option_unknown", check_option(999, 1, kNumTiles, &m));
@@ -148,6 +148,8 @@ int main(int argc, char** argv)
         put(p.block_coef);
         std::printf(", \"ref_coef\": ");
         put(p.ref_coef);
+        std::printf(", \"refs_coef\": ");
+        put(p.refs_coef);
         std::printf(", \"lin_coef\": ");
         put(p.lin_coef);
         std::printf(", \"du_max\": [");
@@ -162,6 +164,16 @@ int main(int argc, char** argv)
             for (int v = 0; v < kNumTiles; ++v) {
                 if (v) std::printf(", ");
                 put(plan_bound(&p, v, rhos[i]));
+            }
+            std::printf("], \"ref_angle\": [");
+            for (int v = 0; v < kNumTiles; ++v) {
+                if (v) std::printf(", ");
+                put(ref_angle_bound(&p, v, rhos[i]));
+            }
+            std::printf("], \"ref_centre\": [");
+            for (int v = 0; v < kNumTiles; ++v) {
+                if (v) std::printf(", ");
+                put(ref_trunc_centre(&p, v, rhos[i]));
             }
             double best = 0.0;
             const std::vector<int> ok = eligible_tiles(&p, rhos[i], &best);

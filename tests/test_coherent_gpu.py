@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 COHERENT_RTOL = 1e-4
 TOL_REV = 5e-6  # the planner's default per-term phase tolerance (IDISIM_OPT_TOL_PPB 5000)
-TILES = [t for t in range(14) if t != 4]  # 4 is a reserved index without a kernel
+TILES = [t for t in range(15) if t != 4]  # 4 is a reserved index without a kernel
 
 
 @pytest.fixture(scope="module")
@@ -136,8 +136,10 @@ def _three_balls(rng, n):
 def test_coherent_near_field_rows_every_eligible_tile(lib, oracle_mod):
     """Near field, phases zero, FP64 oracle rows through the q = 0 peak and the Bragg rows:
     100000 of the simple-cubic sites (0.3 nm) in a 10 nm ball on the 512^2 detector at 0.2 m, a
-    10 x 10 x 10 lattice of 2.1 nm there (Bragg rows 61 and 451), and 250000 emitters in the three
-    balls of the large workload with amplitudes U[0.5, 1] on the 1024^2 detector at 0.5 m."""
+    10 x 10 x 10 lattice of 2.1 nm there (Bragg rows 61 and 451), 250000 emitters in the three
+    balls of the large workload with amplitudes U[0.5, 1] on the 1024^2 detector at 0.5 m, and
+    20000 simple-cubic sites (0.6 nm) in a 30 nm ball on the batched shots' 256^2 detector of
+    100 um pixels at 0.1 m (where the centred tiles 7, 8, 12 and 14 qualify)."""
     import torch
 
     from oracle import Geometry
@@ -153,6 +155,10 @@ def test_coherent_near_field_rows_every_eligible_tile(lib, oracle_mod):
         ("3balls_1024px", Geometry(1024, 1024, 50.0, 5e5, K_UM, nf=True),
          np.c_[_three_balls(rng, 250_000), np.zeros(250_000)], rng.uniform(0.5, 1.0, 250_000), [511, 512, 800]),
     ]
+    sites30 = _sc_sphere(6e-4, 0.03)
+    sc30 = sites30[rng.choice(len(sites30), 20_000, replace=False)]
+    cases.append(("sc20k_30nm_256px", Geometry(256, 256, 100.0, 1e5, K_UM, nf=True), np.c_[sc30, np.zeros(len(sc30))],
+                  None, [0, 127, 128, 200]))
     for name, g, at, amp, rows in cases:
         ref = np.abs(oracle_mod.field_c(g, at, amp, rows=rows)) ** 2
         h = lib.Handle(g.n0, g.n1, g.pixelsize, g.detz, K_UM, nf=True, precision=lib.PREC_FP32SPLIT)

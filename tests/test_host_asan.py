@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 from conftest import K_UM
-from test_planner_cpu import CASES, _block_bound_coef, _bound_coef, _lin_residual, _ref_bound_coef
+from test_planner_cpu import CASES, _block_bound_coef, _bound_coef, _lin_residual, _ref_bound_coef, _ref_true_coef
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TOL = 5e-6
@@ -95,6 +95,12 @@ def test_planner_tables_match_the_numpy_restatement(driver):
     for g, r in zip(small, _plan(driver, small, [0.01])):
         # the FP32 roundings of D and J may fall differently (FP64 formed in another order)
         assert r["lin_coef"] == pytest.approx(1.05 * _lin_residual(g), rel=1e-4)
+    # the exact table of the sub-block reference offset (variant 14): the restatement's maximum with
+    # FP64 references (the compiled table uses the FP32 D_k of the kernel, 1e-5 of the offset apart)
+    mid = [_geom(a) for a, _ in CASES if a["n0"] * a["n1"] <= 512 * 512]
+    for g, r in zip(mid, _plan(driver, mid, [0.01])):
+        assert r["refs_coef"] == pytest.approx(1.05 * _ref_true_coef(g, 8), rel=1e-3), (g.n0, g.n1)
+        assert r["refs_coef"] <= 1.05 * _ref_bound_coef(g, 8)
 
 
 @pytest.mark.parametrize("nf", [False, True])
@@ -106,13 +112,42 @@ def test_planner_bounds_and_eligibility(driver, nf):
             prev = None
             for e in r["extents"]:
                 b = [np.inf if v is None else v for v in e["bound"]]
-                want = [v for v in range(14) if v != 4 and b[v] <= TOL and not (summode and v >= 5)]
+                want = [v for v in range(15) if v != 4 and b[v] <= TOL and not (summode and v >= 5)]
                 assert e["eligible"] == want, (e["rho"], e["eligible"], want)
                 if prev is not None and not nf:
                     # no curvature term to fold: a larger extent gives a larger bound (in the near
                     # field the bound drops where W stops being folded and its fold error with it)
                     assert all(x >= y for x, y in zip(b, prev)), e["rho"]
                 prev = b
+
+
+def test_subblock_scale_halves_the_truncation(driver):
+    """Variant 14 rotates its reference phasors to first order, and a lane whose own angle bound T_k exceeds
+    T / sqrt 2 (T the planner's bound, reported as the centre c = T^2 / 4) scales its sums by
+    1 / (1 + T_k^2 / 2 - c).  Checked on a grid of T_k <= T and |t| <= T_k in FP64: the error of the scaled
+    first-order rotation stays within T^2 / 4 + T^3 / 2, half the T^2 / 2 of the unscaled one, and the lanes
+    with T_k <= T / sqrt 2 are left unscaled."""
+    g = _geom(dict(n0=256, n1=256, pixelsize=100.0, detz=1e5), nf=True)
+    (r,) = _plan(driver, [g], [0.03])
+    e = r["extents"][0]
+    T, c = e["ref_angle"][14], e["ref_centre"][14]
+    assert 1e-3 < T < 1e-2 and c == pytest.approx(T * T / 4, rel=1e-12)
+    assert all(v == 0.0 for i, v in enumerate(e["ref_centre"]) if i != 14)
+    worst, scaled = 0.0, 0
+    for tk in np.linspace(0.0, T, 201):
+        ck = max(0.0, tk * tk / 2 - c)
+        scaled += ck > 0
+        t = np.linspace(-tk, tk, 401)
+        err = np.abs((1 + 1j * t) / (1 + ck) - np.exp(1j * t))
+        worst = max(worst, err.max())
+    assert worst <= T * T / 4 + T**3 / 2
+    assert worst >= 0.9 * T * T / 4  # attained at T_k = T
+    assert 0.25 < scaled / 201 < 0.35  # only T_k > T / sqrt 2
+    t = np.linspace(-T, T, 401)
+    assert np.abs((1 + 1j * t) - np.exp(1j * t)).max() >= 0.99 * T * T / 2
+    # variant 14 is within the tolerance for the batched shots' detector and 30 nm ball, variant 11 (the same
+    # rotation from the 16 x 16 wave reference) is not
+    assert 14 in e["eligible"] and 11 not in e["eligible"]
 
 
 def test_planner_bound_of_the_benchmark_image(driver):

@@ -133,9 +133,10 @@ def test_block_pair_centre_bound(args, slack):
     assert bound <= 1.5 * slack * true, (true, bound)
 
 
-def _ref_bound_coef(g):
-    """1.05 C(s) krev pix^2 |a| |o| / R^2 with |a| = sqrt(2)/2, |o| = 7 sqrt(2) pixels, C(s) = 1 + (2 + sqrt 2) s
-    + 6 s^2, and s, R over the detector padded to whole 32 x 32 tiles (plus one pixel before the first)."""
+def _ref_bound_coef(g, block=16):
+    """1.05 C(s) krev pix^2 |a| |o| / R^2 with |a| = sqrt(2)/2, |o| = (block/2 - 1) sqrt(2) pixels (7 sqrt 2 for the
+    16 x 16 wave blocks, 3 sqrt 2 for the 8 x 8 sub-blocks), C(s) = 1 + (2 + sqrt 2) s + 6 s^2, and s, R over the
+    detector padded to whole 32 x 32 tiles (plus one pixel before the first)."""
     x0 = (-1 - g.n0 / 2) * g.pixelsize + g.detoffset[0]
     x1 = (32 * ((g.n0 + 31) // 32) - g.n0 / 2) * g.pixelsize + g.detoffset[0]
     y0 = (-1 - g.n1 / 2) * g.pixelsize + g.detoffset[1]
@@ -148,26 +149,28 @@ def _ref_bound_coef(g):
     rmin = min(float(np.sqrt(detx**2 + dety**2 + g.detz**2).min()), float(np.sqrt(nx**2 + ny**2 + g.detz**2)))
     s = min(1.0, max(abs(x0), abs(x1), abs(y0), abs(y1)) / rmin)
     return (1.05 * (1 + (2 + np.sqrt(2)) * s + 6 * s * s) * g.k / (2 * np.pi) * g.pixelsize**2
-            * (np.sqrt(2) / 2) * (7 * np.sqrt(2)) / rmin**2)
+            * (np.sqrt(2) / 2) * ((block // 2 - 1) * np.sqrt(2)) / rmin**2)
 
 
-def _ref_true_coef(g):
-    """max over the 16 x 16 pixel wave blocks of 32 x 32 tiles and their quads that hold a detector pixel of
-    krev |d_k(quad) - D_k(block centre)|, d_0 = (dq(1,1) - dq(0,0)) / 2, d_1 = (dq(0,1) - dq(1,0)) / 2."""
+def _ref_true_coef(g, block=16):
+    """max over the block x block pixel blocks of 32 x 32 tiles (16: the wave blocks of variants 9..12; 8: the
+    sub-blocks of variant 14) and their quads that hold a detector pixel of krev |d_k(quad) - D_k(block centre)|,
+    d_0 = (dq(1,1) - dq(0,0)) / 2, d_1 = (dq(0,1) - dq(1,0)) / 2."""
     X, Y = 32 * ((g.n0 + 31) // 32), 32 * ((g.n1 + 31) // 32)
     u = _unit_dirs(g, np.arange(X + 1, dtype=np.float64), np.arange(Y + 1, dtype=np.float64))
     d0 = 0.5 * (u[:, 1:, 1:] - u[:, :-1, :-1])  # quad with lower corner (x, y), shape (3, X, Y)
     d1 = 0.5 * (u[:, :-1, 1:] - u[:, 1:, :-1])
     worst = 0.0
-    for bx in range(0, X, 16):
-        for by in range(0, Y, 16):
-            qx = np.arange(bx, bx + 16, 2)
-            qy = np.arange(by, by + 16, 2)
+    c = block // 2 - 1  # the reference quad's lower corner in the block
+    for bx in range(0, X, block):
+        for by in range(0, Y, block):
+            qx = np.arange(bx, bx + block, 2)
+            qy = np.arange(by, by + block, 2)
             qx, qy = qx[qx < g.n0], qy[qy < g.n1]
             if qx.size == 0 or qy.size == 0:
                 continue
             for d in (d0, d1):
-                D = d[:, bx + 7, by + 7]
+                D = d[:, bx + c, by + c]
                 diff = d[:, qx[:, None], qy[None, :]] - D[:, None, None]
                 worst = max(worst, float(np.linalg.norm(diff, axis=0).max()))
     return g.k / (2 * np.pi) * worst
@@ -184,6 +187,23 @@ def test_wave_reference_bound(args, slack):
     true, bound = _ref_true_coef(g), _ref_bound_coef(g)
     assert true <= bound, (true, bound)
     assert bound <= 1.5 * slack * true, (true, bound)
+
+
+@pytest.mark.parametrize("args,slack", CASES)
+def test_subblock_reference_bound(args, slack):
+    """The analytic bound of |d_k - D_k| against the 8 x 8 pixel sub-block references of variant 14 (the planner's
+    fallback above 2048 x 2048 pixels, where it does not tabulate the exact value) against the exact maximum."""
+    from oracle import Geometry
+
+    g = Geometry(args["n0"], args["n1"], args["pixelsize"], args["detz"], K_UM,
+                 detoffset=args.get("detoffset", (0.0, 0.0)), detangles=args.get("detangles", (0.0, 0.0)))
+    true, bound = _ref_true_coef(g, 8), _ref_bound_coef(g, 8)
+    assert true <= bound, (true, bound)
+    assert bound <= 1.5 * slack * true, (true, bound)
+    # the sub-blocks take the rotation angle to about 3/7 of the wave blocks' (the ratio of the largest
+    # offsets; more where the field of view is wide and d_k far from linear)
+    ratio = true / _ref_true_coef(g, 16)
+    assert ratio <= (0.5 if slack <= 2.0 else 0.6), ratio
 
 
 def test_wave_reference_bound_catches_a_wrong_centre():
