@@ -48,7 +48,7 @@ struct Geom {
     // memory that the host reads on its next call.  viol == nullptr: no check.
     double ext2;
     unsigned int* viol;
-    // Set per launch for tile variant 14: refrho = rho (1 + sigma), the largest |r'_cj|, and
+    // Set per launch for tile variants 14, 15: refrho = rho (1 + sigma), the largest |r'_cj|, and
     // refc = T^2 / 4, T the planner's bound of the rotation angle t = (2 pi (d_k - D_k)) . r'.
     // A lane whose own bound T_k = |2 pi (d_k - D_k)| refrho exceeds T / sqrt 2 scales its pair
     // sums by 1 / (1 + T_k^2 / 2 - refc), which keeps the first-order truncation within T^2 / 4.
@@ -111,9 +111,11 @@ __host__ __device__ inline Frame pixel_frame(const Geom& g, double fx, double fy
 
 // Half differences of the 2 x 2 pixel quad with lower corner (x, y) (fractional pixel
 // indices): h_0 = (dq(x+1, y+1) - dq(x, y)) / 2 and h_1 = (dq(x, y+1) - dq(x+1, y)) / 2,
-// dq = -krev (u - uc), the mirror-pair half differences of the pair tiles.
+// dq = -krev (u - uc), the mirror-pair half differences of the pair tiles; with mm, also
+// m_0 - m_1 = (dq(x, y) + dq(x+1, y+1) - dq(x+1, y) - dq(x, y+1)) / 2, the offset between the
+// centres of the quad's two pairs.
 __host__ __device__ inline void quad_half_diff(const Geom& g, const double uc[3], double x, double y,
-                                               double h[2][3])
+                                               double h[2][3], double* mm = nullptr)
 {
     const Frame f00 = pixel_frame(g, x, y), f11 = pixel_frame(g, x + 1.0, y + 1.0);
     const Frame f10 = pixel_frame(g, x + 1.0, y), f01 = pixel_frame(g, x, y + 1.0);
@@ -122,6 +124,7 @@ __host__ __device__ inline void quad_half_diff(const Geom& g, const double uc[3]
         const double q10 = -g.krev * (f10.u[a] - uc[a]), q01 = -g.krev * (f01.u[a] - uc[a]);
         h[0][a] = 0.5 * (q11 - q00);
         h[1][a] = 0.5 * (q01 - q10);
+        if (mm != nullptr) mm[a] = 0.5 * (q00 + q11) - 0.5 * (q10 + q01);
     }
 }
 
@@ -230,7 +233,10 @@ struct WorkItem {
 // rotation angle is about 3/7 as large, and the lanes with the largest angles scale
 // their sums so that the first-order truncation is at most T^2 / 4 instead of T^2 / 2;
 // the first order then qualifies where 11 needs the second order of 12.
-constexpr int kNumTiles = 15;
+// 15: as 14, with the rotation between the two pair centres of a quad (the centring of 7)
+// also taken from the sub-block's reference quad, formed once per emitter and sub-block
+// instead of once per lane: 3 fewer FMA per quad, for a residual the planner tabulates.
+constexpr int kNumTiles = 16;
 constexpr int kTileNone = 4;  // reserved index, no kernel
 constexpr int kTilePair1 = 5;
 constexpr int kTilePair2 = 6;
@@ -242,18 +248,21 @@ constexpr int kTileRefC1 = 11;
 constexpr int kTileRefC2 = 12;
 constexpr int kTileRefL = 13;
 constexpr int kTileRefS = 14;
-__host__ __device__ constexpr bool tile_pair(int v) { return v >= kTilePair1 && v <= kTileRefS; }
+constexpr int kTileRefSC = 15;
+__host__ __device__ constexpr bool tile_pair(int v) { return v >= kTilePair1 && v <= kTileRefSC; }
 __host__ __device__ constexpr bool tile_pair_centred(int v)
 {
-    return v == kTilePairC || v == kTileRefC1 || v == kTileRefC2 || v == kTileRefS;
+    return v == kTilePairC || v == kTileRefC1 || v == kTileRefC2 || v == kTileRefS || v == kTileRefSC;
 }
 // kind of wave reference: 0 none (own sincos), 1 first-order rotation, 2 second order,
 // 3 first order with the angle linear in the quad offset, 4 first order from a reference
-// per 8 x 8 pixel sub-block with magnitude-centred phasors
+// per 8 x 8 pixel sub-block (the far lanes' sums scaled), 5 as 4 with the pair-centre
+// rotation from the sub-block reference
 __host__ __device__ constexpr int tile_ref_kind(int v)
 {
     return (v == kTileRef1 || v == kTileRefC1) ? 1
-           : ((v == kTileRef2 || v == kTileRefC2) ? 2 : (v == kTileRefL ? 3 : (v == kTileRefS ? 4 : 0)));
+           : ((v == kTileRef2 || v == kTileRefC2) ? 2
+                                                  : (v == kTileRefL ? 3 : (v == kTileRefS ? 4 : (v == kTileRefSC ? 5 : 0))));
 }
 // order of the rotation from the reference to the lane's delta_k (0: own sincos)
 __host__ __device__ constexpr int tile_ref_order(int v)

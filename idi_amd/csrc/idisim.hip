@@ -208,7 +208,7 @@ Geom split_geom(const idisim_handle* h, int tile, bool* w)
     const double e = h->rho * (1.0 + 1e-9);
     gl.ext2 = e * e;
     gl.viol = h->d_viol;
-    if (tile_ref_kind(tile) == 4) {
+    if (tile_ref_kind(tile) >= 4) {
         gl.refrho = (float)(h->rho * (1.0 + sigma_bound(h, tile, h->rho)));
         gl.refc = (float)ref_trunc_centre(h, tile, h->rho);
     }
@@ -371,7 +371,7 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
     } else {
         // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
-        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRefS, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
+        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRefSC, kTileRefS, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;
@@ -1063,7 +1063,7 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
         h->tuned[key] = ch;
     } else {
         ch.tile = tiles.front();
-        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRefS, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
+        for (int v : {kTileRefL, kTileRef1, kTileRefC1, kTileRefSC, kTileRefS, kTileRef2, kTileRefC2, kTilePair1, kTileBlock, kTilePairC, 2, 1, 3, 0})
             if (has_tile(tiles, v)) {
                 ch.tile = v;
                 break;

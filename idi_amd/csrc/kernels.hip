@@ -265,15 +265,23 @@ constexpr int kSubStride = 17;
 // REF 3 (variant 13) holds two float4 per emitter and wave: 224 emitters per round keep a
 // workgroup's LDS under 32 KB, so that five workgroups share a CU.
 constexpr int kRef3Chunk = 224;
+// REF 5 (variant 15) adds one float per emitter and sub-block, the rotation between the pair
+// centres, 2 pi (m_0 - m_1) . r' of the sub-block's reference quad: 112 emitters per round.
+constexpr int kRef5Chunk = 112;
 __host__ __device__ constexpr int pair_chunk(int ref)
 {
-    return ref == 4 ? 128 : (ref == 3 ? kRef3Chunk : kPairChunk);
+    return ref == 4 ? 128 : (ref == 5 ? kRef5Chunk : (ref == 3 ? kRef3Chunk : kPairChunk));
 }
-__host__ __device__ constexpr int pair_sd_floats(int ref) { return ref == 4 ? 6 * kSubRefs : (ref ? 72 : 1); }
+// per sub-block: D_0, D_1 (REF 4), and 2 pi (m_0 - m_1) (REF 5)
+__host__ __device__ constexpr int pair_sd_floats(int ref)
+{
+    return ref == 5 ? 9 * kSubRefs : (ref == 4 ? 6 * kSubRefs : (ref ? 72 : 1));
+}
 __host__ __device__ constexpr int pair_se_float4(int ref)
 {
-    return ref == 0 ? 1 : (ref == 1 ? 4 * kPairChunk : (ref == 4 ? kSubStride * 128 : 8 * pair_chunk(ref)));
+    return ref == 0 ? 1 : (ref == 1 ? 4 * kPairChunk : (ref >= 4 ? kSubStride * pair_chunk(ref) : 8 * pair_chunk(ref)));
 }
+__host__ __device__ constexpr int pair_sc_floats(int ref) { return ref == 5 ? kSubRefs * kRef5Chunk : 1; }
 // (W, amplitude) of each staged emitter: read only by the tiles without a reference (the
 // others fold the curvature into A and the amplitude into their reference phasors)
 __host__ __device__ constexpr int pair_s1_float2(int ref) { return ref == 0 ? kPairChunk : 1; }
@@ -294,7 +302,7 @@ __device__ __forceinline__ void pair_lane(int tid, int& lx, int& ly)
 template <int QY, bool HAS_AMP, bool CENTRED, int REF>
 __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restrict__ emit,
                                           const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0,
-                                          int ty0, float4* s0, float2* s1, float* sD, float4* sE,
+                                          int ty0, float4* s0, float2* s1, float* sD, float4* sE, float* sC,
                                           float (&acc)[QY][2][4], float (&cmp)[QY][2][4])
 {
     constexpr int TX = 32, TY = 32 * QY;
@@ -340,20 +348,22 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             }
         }
     }
-    int myref = 0;  // REF 4: the lane's sub-block, 4 w + (2 x half + y half of the wave block)
-    if (REF == 1 || REF == 2 || REF == 4) {
-        static_assert(REF != 4 || (QY == 1 && CENTRED), "sub-block references: one centred quad per lane");
+    constexpr bool SUB = REF == 4 || REF == 5;  // a reference per 8 x 8 pixel sub-block
+    constexpr int DSTR = REF == 5 ? 9 : 6;       // floats per sub-block in sD
+    int myref = 0;  // REF 4, 5: the lane's sub-block, 4 w + (2 x half + y half of the wave block)
+    if (REF == 1 || REF == 2 || SUB) {
+        static_assert(!SUB || (QY == 1 && CENTRED), "sub-block references: one centred quad per lane");
         const int w = tid >> 6, l = tid & 63;
         const int sb = ((l >> 5) << 1) | ((l >> 2) & 1);
         myref = 4 * w + sb;
         // the reference quad's centre: the wave block's centre (REF 1, 2) or the sub-block's
-        const double px = REF == 4 ? (double)(tx0 + 16 * (w >> 1) + 8 * (sb >> 1)) + 3.5
-                                   : (double)(tx0 + 16 * (w >> 1)) + 7.5;
-        const double py = REF == 4 ? (double)(ty0 + 16 * (w & 1) + 8 * (sb & 1)) + 3.5
-                                   : (double)(ty0 + 16 * QY * (w & 1)) + (8.0 * QY - 0.5);
+        const double px = SUB ? (double)(tx0 + 16 * (w >> 1) + 8 * (sb >> 1)) + 3.5
+                              : (double)(tx0 + 16 * (w >> 1)) + 7.5;
+        const double py = SUB ? (double)(ty0 + 16 * (w & 1) + 8 * (sb & 1)) + 3.5
+                              : (double)(ty0 + 16 * QY * (w & 1)) + (8.0 * QY - 0.5);
         // the lane that stores its reference: the first of the wave (REF 1, 2) or of the sub-block
-        const bool writer = REF == 4 ? (l & 27) == 0 : l == 0;
-        float* Dst = REF == 4 ? sD + 6 * myref : sD + 18 * w;
+        const bool writer = SUB ? (l & 27) == 0 : l == 0;
+        float* Dst = SUB ? sD + DSTR * myref : sD + 18 * w;
         const Frame f00 = pixel_frame(g, px - 0.5, py - 0.5), f11 = pixel_frame(g, px + 0.5, py + 0.5);
         const Frame f10 = pixel_frame(g, px + 0.5, py - 0.5), f01 = pixel_frame(g, px - 0.5, py + 0.5);
 #pragma unroll
@@ -366,6 +376,8 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             if (writer) {
                 Dst[a] = D0;
                 Dst[3 + a] = D1;
+                // REF 5: the reference quad's 2 pi (m_0 - m_1), as the lanes' dv below
+                if (REF == 5) Dst[6 + a] = (float)(kTwoPi * (0.5 * (q00 + q11) - 0.5 * (q10 + q01)));
             }
         }
     }
@@ -406,29 +418,29 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
         }
     }
     constexpr int CH = pair_chunk(REF);
-    // this lane's phasors: its wave's (REF 1..3, emitter stride 1) or its sub-block's (REF 4)
-    constexpr int ESTR = REF == 4 ? kSubStride : 1;
-    const float4* eref = REF == 4 ? sE + myref : sE + ((REF == 1 ? 1 : 2) * (tid >> 6)) * CH;
+    // this lane's phasors: its wave's (REF 1..3, emitter stride 1) or its sub-block's (REF 4, 5)
+    constexpr int ESTR = SUB ? kSubStride : 1;
+    const float4* eref = SUB ? sE + myref : sE + ((REF == 1 ? 1 : 2) * (tid >> 6)) * CH;
 
     for (int64_t base = e0; base < e1; base += CH) {
         const int cnt = (e1 - base) < (int64_t)CH ? (int)(e1 - base) : CH;
-        // REF 4: lanes tid and tid + 128 both form the record of emitter tid and each stores 8 of
-        // its 16 sub-block phasors
-        const int el = REF == 4 ? (tid & (CH - 1)) : tid;
+        // REF 4, 5: lanes tid and tid + 128 both form the record of emitter tid and each stores 8
+        // of its 16 sub-block phasors
+        const int el = SUB ? (tid & 127) : tid;
         __syncthreads();  // the previous chunk has been consumed by every lane
         if (el < cnt) {
             const int64_t e = base + el;
             const double* a4 = emit + 4 * e;
             const double ea = (HAS_AMP && amp != nullptr) ? amp[e] : 1.0;
             const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], ea);
-            if (REF != 4 || tid < CH) s0[el] = make_float4(r.x, r.y, r.z, r.A);
+            if (!SUB || tid < 128) s0[el] = make_float4(r.x, r.y, r.z, r.A);
             if (REF == 0) s1[el] = make_float2(r.W, r.amp);
-            if (REF == 4) {
+            if (SUB) {
                 const float sc = HAS_AMP ? r.amp : 1.f;
                 const int first = (tid >> 7) * (kSubRefs / 2);
 #pragma unroll 2
                 for (int j = 0; j < kSubRefs / 2; ++j) {
-                    const float* D = sD + 6 * (first + j);
+                    const float* D = sD + DSTR * (first + j);
                     float dl0 = D[0] * r.x, dl1 = D[3] * r.x;
                     dl0 = fmaf(D[1], r.y, dl0);
                     dl1 = fmaf(D[4], r.y, dl1);
@@ -437,6 +449,12 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                     sE[el * kSubStride + first + j] =
                         make_float4(sc * __builtin_amdgcn_cosf(dl0), sc * __builtin_amdgcn_sinf(dl0),
                                     sc * __builtin_amdgcn_cosf(dl1), sc * __builtin_amdgcn_sinf(dl1));
+                    if (REF == 5) {
+                        float dc = D[6] * r.x;
+                        dc = fmaf(D[7], r.y, dc);
+                        dc = fmaf(D[8], r.z, dc);
+                        sC[el * kSubRefs + first + j] = dc;
+                    }
                 }
             } else if (REF == 3) {
 #pragma unroll
@@ -495,7 +513,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             for (int k = 0; k < 2; ++k)
 #pragma unroll
                 for (int m = 0; m < 4; ++m) su[q][k][m] = 0.f;
-#pragma unroll 2
+#pragma unroll (SUB ? 4 : 2)  // SUB: one address update per four emitters (the VGPR bound holds)
         for (int e = 0; e < cnt; ++e) {
             const float4 r0 = s0[e];
             const float am = (HAS_AMP && REF == 0) ? s1[e].y : 1.f;
@@ -515,9 +533,14 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                 }
                 float pc[2] = {cc, cc}, ps[2] = {sc, sc};  // centre phasor of each pair
                 if (CENTRED) {
-                    float dl = dv[q][0] * r0.x;
-                    dl = fmaf(dv[q][1], r0.y, dl);
-                    dl = fmaf(dv[q][2], r0.z, dl);
+                    float dl;
+                    if (REF == 5) {  // the sub-block reference's rotation between the pair centres
+                        dl = sC[e * kSubRefs + myref];
+                    } else {
+                        dl = dv[q][0] * r0.x;
+                        dl = fmaf(dv[q][1], r0.y, dl);
+                        dl = fmaf(dv[q][2], r0.z, dl);
+                    }
                     pc[1] = fmaf(dl, sc, cc);   // (cc + i sc)(1 - i dl)
                     ps[1] = fmaf(-dl, cc, sc);
                 }
@@ -561,7 +584,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
 #pragma unroll
                 for (int m = 0; m < 4; ++m) kahan_add(acc[q][k][m], cmp[q][k][m], su[q][k][m]);
     }
-    if (REF == 4) {
+    if (SUB) {
         // the lane's own angle bound T_k = |2 pi (d_k - D_k)| max|r'|; beyond T / sqrt 2 its sums
         // take the factor 1 / (1 + T_k^2 / 2 - T^2 / 4) (the planner's truncation bound)
 #pragma unroll
@@ -748,13 +771,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
     __shared__ float2 s1[pair_s1_float2(REF)];
     __shared__ float sD[pair_sd_floats(REF)];
     __shared__ float4 sE[pair_se_float4(REF)];
+    __shared__ float sC[pair_sc_floats(REF)];
     const int tile = blockIdx.x;
     const int split = blockIdx.y;
     const int tx0 = (tile / tiles_y) * TX, ty0 = (tile % tiles_y) * TY;
     const int64_t e0 = (int64_t)split * per_split;
     const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
     float acc[QY][2][4], cmp[QY][2][4];
-    pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, sD, sE, acc, cmp);
+    pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, sD, sE, sC, acc, cmp);
 
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);
@@ -867,9 +891,11 @@ __global__ __launch_bounds__(kBlock) void k_batched_block(Geom g0, const double*
     }
 }
 
-// Persistent batched form of k_fwd_pair (the queue protocol of k_batched_split).
+// Persistent batched form of k_fwd_pair (the queue protocol of k_batched_split).  The one-quad
+// variants are held to 128 VGPRs (four waves per SIMD, as their LDS allows): without the bound
+// the allocator settles a few registers above it and one wave per SIMD is lost.
 template <int QY, bool HAS_AMP, bool CENTRED, int REF>
-__global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g0, const double* __restrict__ emit,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QY == 1 ? 4 : 2))) void k_batched_pair(Geom g0, const double* __restrict__ emit,
                                                          const double* __restrict__ amp,
                                                          const WorkItem* __restrict__ items, int nitems,
                                                          int* __restrict__ counter, int tiles_y, void* out, int kind)
@@ -879,6 +905,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g0, const double* 
     __shared__ float2 s1[pair_s1_float2(REF)];
     __shared__ float sD[pair_sd_floats(REF)];
     __shared__ float4 sE[pair_se_float4(REF)];
+    __shared__ float sC[pair_sc_floats(REF)];
     __shared__ int s_item;
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);
@@ -893,7 +920,7 @@ __global__ __launch_bounds__(kBlock) void k_batched_pair(Geom g0, const double* 
         const Geom g = item_geom();
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         float acc[QY][2][4], cmp[QY][2][4];
-        pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, sD, sE, acc, cmp);
+        pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, sD, sE, sC, acc, cmp);
         char* outs = static_cast<char*>(out) +
                      (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
         // read again, so that the pixel frames of the stores are formed here and not kept in
@@ -1308,6 +1335,8 @@ static int occupancy_pair_q()
         case 45: return FN<1, true, false, 3>(__VA_ARGS__);                                   \
         case 50: return FN<1, false, true, 4>(__VA_ARGS__);                                   \
         case 51: return FN<1, true, true, 4>(__VA_ARGS__);                                    \
+        case 54: return FN<1, false, true, 5>(__VA_ARGS__);                                   \
+        case 55: return FN<1, true, true, 5>(__VA_ARGS__);                                    \
         case 64: return FN<2, false, false, 0>(__VA_ARGS__);                                  \
         case 65: return FN<2, true, false, 0>(__VA_ARGS__);                                   \
         default: break;                                                                       \

@@ -35,7 +35,8 @@ struct Planner {
     double pair_coef = 0.0;   // mirror-pair remainder per unit emitter extent (rev per length)
     double block_coef = 0.0;  // bound of |m_k - m_1| of the 2 x 4 block pairs per unit extent
     double ref_coef = 0.0;    // bound of |d_k - D_k| (lane pair vs wave reference) per unit extent
-    double refs_coef = 0.0;   // the same against the 8 x 8 pixel sub-block reference (variant 14)
+    double refs_coef = 0.0;   // the same against the 8 x 8 pixel sub-block reference (variants 14, 15)
+    double cent_coef = 0.0;   // largest |(m_0 - m_1) - (m_0 - m_1) of the sub-block reference| (variant 15)
     double lin_coef = 0.0;    // largest residual of the linear wave reference (variant 13) per unit extent
 };
 
@@ -141,12 +142,15 @@ inline void planner_tables(Planner* h)
     // wave_linear_ref), and its largest norm (with 5% margin for the FP64 rounding of this
     // evaluation) enters the bound.  Detectors beyond 2048 x 2048 pixels skip the table and the
     // variant.
-    // The sub-block references of variant 14 are tabulated in the same pass: the largest norm of
-    // d_k - D_k over every quad, D_k the FP32 half differences of the sub-block's reference quad
-    // (lower corner 3 pixels into the 8 x 8 sub-block) as the kernel forms them.
+    // The sub-block references of variants 14 and 15 are tabulated in the same pass: the largest
+    // norm of d_k - D_k over every quad, D_k the FP32 half differences of the sub-block's reference
+    // quad (lower corner 3 pixels into the 8 x 8 sub-block) as the kernel forms them, and for 15
+    // the largest norm of the quad's m_0 - m_1 against the reference quad's (FP32 2 pi (m_0 - m_1)
+    // as the kernel forms it).  Above 2048 x 2048 pixels variant 15 is not offered.
     h->lin_coef = std::numeric_limits<double>::infinity();
+    h->cent_coef = std::numeric_limits<double>::infinity();
     if ((int64_t)g.n0 * g.n1 <= (int64_t)2048 * 2048) {
-        double worst = 0.0, worst_sub = 0.0;
+        double worst = 0.0, worst_sub = 0.0, worst_cent = 0.0;
         for (int tx0 = 0; tx0 < g.n0; tx0 += 32) {
             for (int ty0 = 0; ty0 < g.n1; ty0 += 32) {
                 const int xe = std::min(tx0 + 32, g.n0) - 1, ye = std::min(ty0 + 32, g.n1) - 1;
@@ -157,16 +161,19 @@ inline void planner_tables(Planner* h)
                     float D[2][3], Jx[2][3], Jy[2][3];
                     wave_linear_ref(g, c.u, bx + 7.5, by + 7.5, D, Jx, Jy);
                     float Ds[4][2][3];
+                    double Ms[4][3];
                     for (int sb = 0; sb < 4; ++sb) {
-                        double hs[2][3];
-                        quad_half_diff(g, c.u, (double)(bx + 8 * (sb >> 1) + 3), (double)(by + 8 * (sb & 1) + 3), hs);
+                        double hs[2][3], ms[3];
+                        quad_half_diff(g, c.u, (double)(bx + 8 * (sb >> 1) + 3), (double)(by + 8 * (sb & 1) + 3), hs,
+                                       ms);
                         for (int k = 0; k < 2; ++k)
                             for (int a = 0; a < 3; ++a) Ds[sb][k][a] = (float)hs[k][a];
+                        for (int a = 0; a < 3; ++a) Ms[sb][a] = (double)(float)(kTwoPi * ms[a]) * kInv2Pi;
                     }
                     for (int i = 0; i < 8 && bx + 2 * i < g.n0; ++i) {
                         for (int j = 0; j < 8 && by + 2 * j < g.n1; ++j) {
-                            double hq[2][3];
-                            quad_half_diff(g, c.u, (double)(bx + 2 * i), (double)(by + 2 * j), hq);
+                            double hq[2][3], mq[3];
+                            quad_half_diff(g, c.u, (double)(bx + 2 * i), (double)(by + 2 * j), hq, mq);
                             const double ox = 2 * i - 7, oy = 2 * j - 7;
                             const int sb = ((i >> 2) << 1) | (j >> 2);
                             for (int k = 0; k < 2; ++k) {
@@ -181,6 +188,9 @@ inline void planner_tables(Planner* h)
                                 worst = std::max(worst, std::sqrt(n2));
                                 worst_sub = std::max(worst_sub, std::sqrt(s2));
                             }
+                            double c2 = 0.0;
+                            for (int a = 0; a < 3; ++a) c2 += (mq[a] - Ms[sb][a]) * (mq[a] - Ms[sb][a]);
+                            worst_cent = std::max(worst_cent, std::sqrt(c2));
                         }
                     }
                 }
@@ -188,6 +198,7 @@ inline void planner_tables(Planner* h)
         }
         h->lin_coef = 1.05 * worst;
         h->refs_coef = 1.05 * worst_sub;
+        h->cent_coef = 1.05 * worst_cent;
     }
 }
 
@@ -221,11 +232,11 @@ inline double ref_angle_bound(const Planner* h, int v, double rho)
 {
     const int kind = tile_ref_kind(v);
     if (kind == 0) return 0.0;
-    const double coef = kind == 4 ? h->refs_coef : h->ref_coef + (kind == 3 ? h->lin_coef : 0.0);
+    const double coef = kind >= 4 ? h->refs_coef : h->ref_coef + (kind == 3 ? h->lin_coef : 0.0);
     return kTwoPi * coef * rho * (1.0 + sigma_bound(h, v, rho));
 }
 
-// Variant 14's truncation centre T^2 / 4, T the angle bound (0 for the other variants).  A lane
+// The truncation centre T^2 / 4 of variants 14 and 15, T the angle bound (0 for the others).  A lane
 // whose own bound T_k (|t| <= T_k <= T over its emitters) exceeds T / sqrt 2 scales its sums by
 // s = 1 / (1 + c), c = T_k^2 / 2 - T^2 / 4 in (0, T^2 / 4]; then s (1 + i t) - e^{i t} has real
 // part t^2 / 2 - c + O(T^4), within [-T^2 / 4, T^2 / 4], and imaginary part at most 5 T^3 / 12.
@@ -233,7 +244,7 @@ inline double ref_angle_bound(const Planner* h, int v, double rho)
 // instead of T^2 / 2, and only the lanes far from their reference carry the scale's bias.
 inline double ref_trunc_centre(const Planner* h, int v, double rho)
 {
-    if (tile_ref_kind(v) != 4) return 0.0;
+    if (tile_ref_kind(v) < 4) return 0.0;
     const double T = ref_angle_bound(h, v, rho);
     return std::isfinite(T) ? 0.25 * T * T : 0.0;
 }
@@ -304,10 +315,13 @@ inline double plan_bound(const Planner* h, int v, double rho)
             const int kind = tile_ref_kind(v);
             const bool lin = kind == 3;
             const double t = ref_angle_bound(h, v, rho);
-            const double trunc = kind == 4 ? 0.25 * t * t + 0.5 * t * t * t + u24
+            const double trunc = kind >= 4 ? 0.25 * t * t + 0.5 * t * t * t + u24
                                            : (ord == 1 ? 0.5 * t * t : t * t * t / 6.0);
             e += (trunc + (lin ? 8.0 : 4.0) * u24 * t) * kInv2Pi + 2.0 * u24 * B + 2.0 * u24;
             if (lin) e += h->lin_coef * rho * (1.0 + sig);
+            // variant 15 rotates pair 1 by the sub-block reference's (m_0 - m_1) . r' in place of
+            // the quad's own: the difference is a phase error of pair 1
+            if (kind == 5) e += h->cent_coef * rho * (1.0 + sig);
         }
     }
     return e;

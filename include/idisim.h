@@ -81,7 +81,10 @@ extern "C" {
                                      second (10, 12) order, 13 as 9 with the
                                      rotation angle linear in the quad offset,
                                      14 as 11 with a reference per 8x8 px
-                                     sub-block (FP32-split only, 32x32 px);
+                                     sub-block, 15 as 14 with the rotation
+                                     between the pair centres also from the
+                                     sub-block reference (FP32-split only,
+                                     32x32 px);
                                      FP64 handles take 0..3                      */
 #define IDISIM_OPT_SPLIT        3 /* force emitter split count (0 = planner)        */
 #define IDISIM_OPT_SUMMODE      4 /* 0 (default): per-chunk partials merged with

@@ -150,6 +150,8 @@ int main(int argc, char** argv)
         put(p.ref_coef);
         std::printf(", \"refs_coef\": ");
         put(p.refs_coef);
+        std::printf(", \"cent_coef\": ");
+        put(p.cent_coef);
         std::printf(", \"lin_coef\": ");
         put(p.lin_coef);
         std::printf(", \"du_max\": [");

@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 COHERENT_RTOL = 1e-4
 TOL_REV = 5e-6  # the planner's default per-term phase tolerance (IDISIM_OPT_TOL_PPB 5000)
-TILES = [t for t in range(15) if t != 4]  # 4 is a reserved index without a kernel
+TILES = [t for t in range(16) if t != 4]  # 4 is a reserved index without a kernel
 
 
 @pytest.fixture(scope="module")
@@ -139,7 +139,7 @@ def test_coherent_near_field_rows_every_eligible_tile(lib, oracle_mod):
     10 x 10 x 10 lattice of 2.1 nm there (Bragg rows 61 and 451), 250000 emitters in the three
     balls of the large workload with amplitudes U[0.5, 1] on the 1024^2 detector at 0.5 m, and
     20000 simple-cubic sites (0.6 nm) in a 30 nm ball on the batched shots' 256^2 detector of
-    100 um pixels at 0.1 m (where the centred tiles 7, 8, 12 and 14 qualify)."""
+    100 um pixels at 0.1 m (where the centred tiles 7, 8, 12, 14 and 15 qualify)."""
     import torch
 
     from oracle import Geometry

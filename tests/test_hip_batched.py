@@ -56,7 +56,7 @@ def test_batched_equals_single_shot(lib, oracle_mod, precision):
         assert rel_l2(out[s], ref) <= (1e-4 if precision == 1 else fp64_tol(g))
 
 
-@pytest.mark.parametrize("tile", [5, 6, 7, 8, 9, 10, 11, 12, 13, 14])
+@pytest.mark.parametrize("tile", [5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15])
 def test_batched_pair_tiles_equal_single_shot(lib, oracle_mod, tile):
     """Mirror-pair tiles in the persistent batched kernel: bit-identical to the single-shot
     kernel of the same tile (no split), within 1e-4 of the oracle (75 um at 0.2 m, 10 nm balls,
@@ -140,10 +140,10 @@ def test_batched_fallback_coarse_geometry(lib, oracle_mod):
         assert rel_l2(out[s], oracle_mod.field_c(g, shots[s])) <= max(fp64_tol(g), 1e-6)
 
 
-@pytest.mark.parametrize("tile", [None, 7, 8, 12, 14])
+@pytest.mark.parametrize("tile", [None, 7, 8, 12, 14, 15])
 def test_batched_config_shape_rows(lib, oracle_mod, tile):
     """256 x 256 px of 100 um at 0.1 m, shots of N_s ~ U{1000..50000} in a 30 nm ball (a subset of the 256);
-    the planner's choice and the mirror-pair tiles with a centre per pair (7, 8, 12, 14)."""
+    the planner's choice and the mirror-pair tiles with a centre per pair (7, 8, 12, 14, 15)."""
     import torch
 
     from oracle import Geometry

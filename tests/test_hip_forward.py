@@ -71,7 +71,7 @@ def test_golden_split_every_tile_within_its_bound(lib, golden):
         amp = None if case["amp"] is None else np.array(case["amp"])
         ref = golden_field(case)
         scale = golden_term_scale(case)
-        for tile in range(15):
+        for tile in range(16):
             try:
                 f, h = run(lib, g, atoms, amp, tile=tile, split=1)
             except lib.IdisimError:
@@ -203,7 +203,7 @@ def test_empty_and_errors(lib):
     with pytest.raises(ValueError):
         h.forward(torch.zeros((4, 3), dtype=torch.float64, device="cuda"))
     with pytest.raises(lib.IdisimError):
-        h.set_option(lib.OPT_TILE, 15)  # past the last variant
+        h.set_option(lib.OPT_TILE, 16)  # past the last variant
     with pytest.raises(lib.IdisimError):
         h.set_option(lib.OPT_TILE, 4)  # reserved, no kernel
     with pytest.raises(lib.IdisimError):

@@ -22,7 +22,8 @@ import numpy as np
 import pytest
 
 from conftest import K_UM
-from test_planner_cpu import CASES, _block_bound_coef, _bound_coef, _lin_residual, _ref_bound_coef, _ref_true_coef
+from test_planner_cpu import (CASES, _block_bound_coef, _bound_coef, _cent_true_coef, _lin_residual, _ref_bound_coef,
+                               _ref_true_coef)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TOL = 5e-6
@@ -101,6 +102,10 @@ def test_planner_tables_match_the_numpy_restatement(driver):
     for g, r in zip(mid, _plan(driver, mid, [0.01])):
         assert r["refs_coef"] == pytest.approx(1.05 * _ref_true_coef(g, 8), rel=1e-3), (g.n0, g.n1)
         assert r["refs_coef"] <= 1.05 * _ref_bound_coef(g, 8)
+        # the pair-centre residual of variant 15 (its FP32 reference rounding is 1e-7 of m_0 - m_1,
+        # which is up to 10x the residual: compared to 1e-4 of the restatement's value plus that)
+        assert r["cent_coef"] == pytest.approx(1.05 * _cent_true_coef(g), rel=1e-4, abs=1e-5 * r["cent_coef"]), (
+            g.n0, g.n1)
 
 
 @pytest.mark.parametrize("nf", [False, True])
@@ -112,7 +117,7 @@ def test_planner_bounds_and_eligibility(driver, nf):
             prev = None
             for e in r["extents"]:
                 b = [np.inf if v is None else v for v in e["bound"]]
-                want = [v for v in range(15) if v != 4 and b[v] <= TOL and not (summode and v >= 5)]
+                want = [v for v in range(16) if v != 4 and b[v] <= TOL and not (summode and v >= 5)]
                 assert e["eligible"] == want, (e["rho"], e["eligible"], want)
                 if prev is not None and not nf:
                     # no curvature term to fold: a larger extent gives a larger bound (in the near
@@ -132,7 +137,8 @@ def test_subblock_scale_halves_the_truncation(driver):
     e = r["extents"][0]
     T, c = e["ref_angle"][14], e["ref_centre"][14]
     assert 1e-3 < T < 1e-2 and c == pytest.approx(T * T / 4, rel=1e-12)
-    assert all(v == 0.0 for i, v in enumerate(e["ref_centre"]) if i != 14)
+    assert e["ref_centre"][15] == c and e["ref_angle"][15] == T
+    assert all(v == 0.0 for i, v in enumerate(e["ref_centre"]) if i not in (14, 15))
     worst, scaled = 0.0, 0
     for tk in np.linspace(0.0, T, 201):
         ck = max(0.0, tk * tk / 2 - c)
@@ -145,9 +151,11 @@ def test_subblock_scale_halves_the_truncation(driver):
     assert 0.25 < scaled / 201 < 0.35  # only T_k > T / sqrt 2
     t = np.linspace(-T, T, 401)
     assert np.abs((1 + 1j * t) - np.exp(1j * t)).max() >= 0.99 * T * T / 2
-    # variant 14 is within the tolerance for the batched shots' detector and 30 nm ball, variant 11 (the same
-    # rotation from the 16 x 16 wave reference) is not
-    assert 14 in e["eligible"] and 11 not in e["eligible"]
+    # variants 14 and 15 are within the tolerance for the batched shots' detector and 30 nm ball, variant 11
+    # (the same rotation from the 16 x 16 wave reference) is not; 15 adds its pair-centre residual to 14's bound
+    assert 14 in e["eligible"] and 15 in e["eligible"] and 11 not in e["eligible"]
+    # (times 1 + sigma, sigma ~ rho / R = 3e-7 here)
+    assert e["bound"][15] - e["bound"][14] == pytest.approx(r["cent_coef"] * 0.03, rel=1e-5)
 
 
 def test_planner_bound_of_the_benchmark_image(driver):

@@ -206,6 +206,48 @@ def test_subblock_reference_bound(args, slack):
     assert ratio <= (0.5 if slack <= 2.0 else 0.6), ratio
 
 
+def _cent_true_coef(g):
+    """Variant 15 takes the rotation between a quad's two pair centres, m_0 - m_1 = (dq(0,0) + dq(1,1) - dq(1,0)
+    - dq(0,1)) / 2, from the reference quad of its 8 x 8 pixel sub-block (lower corner 3 pixels in): the
+    largest krev |(m_0 - m_1) - (m_0 - m_1)_ref| over the quads that hold a detector pixel."""
+    X, Y = 32 * ((g.n0 + 31) // 32), 32 * ((g.n1 + 31) // 32)
+    u = _unit_dirs(g, np.arange(X + 1, dtype=np.float64), np.arange(Y + 1, dtype=np.float64))
+    mm = 0.5 * (u[:, :-1, :-1] + u[:, 1:, 1:] - u[:, 1:, :-1] - u[:, :-1, 1:])  # quad with lower corner (x, y)
+    worst = 0.0
+    for bx in range(0, X, 8):
+        for by in range(0, Y, 8):
+            qx = np.arange(bx, bx + 8, 2)
+            qy = np.arange(by, by + 8, 2)
+            qx, qy = qx[qx < g.n0], qy[qy < g.n1]
+            if qx.size == 0 or qy.size == 0:
+                continue
+            diff = mm[:, qx[:, None], qy[None, :]] - mm[:, bx + 3, by + 3][:, None, None]
+            worst = max(worst, float(np.linalg.norm(diff, axis=0).max()))
+    return g.k / (2 * np.pi) * worst
+
+
+@pytest.mark.parametrize("args,slack", CASES[:4])
+def test_pair_centre_residual_is_small(args, slack):
+    """On the benchmark detectors the quads' pair-centre offset m_0 - m_1 varies across an 8 x 8 pixel sub-block
+    by a small part of its own size (what variant 15 drops), and that part shrinks with the sub-block: at most
+    a third of the variation across a whole 32 x 32 tile."""
+    from oracle import Geometry
+
+    g = Geometry(args["n0"], args["n1"], args["pixelsize"], args["detz"], K_UM)
+    X = 32 * ((g.n0 + 31) // 32)
+    u = _unit_dirs(g, np.arange(X + 1, dtype=np.float64), np.arange(X + 1, dtype=np.float64))
+    mm = 0.5 * (u[:, :-1, :-1] + u[:, 1:, 1:] - u[:, 1:, :-1] - u[:, :-1, 1:])
+    size = g.k / (2 * np.pi) * float(np.linalg.norm(mm[:, :g.n0, :g.n1], axis=0).max())
+    tile = 0.0
+    for bx in range(0, X, 32):
+        for by in range(0, X, 32):
+            d = mm[:, bx:bx + 32:2, by:by + 32:2] - mm[:, bx + 15, by + 15][:, None, None]
+            tile = max(tile, float(np.linalg.norm(d, axis=0).max()))
+    sub = _cent_true_coef(g)
+    assert sub <= 0.1 * size, (sub, size)
+    assert sub <= g.k / (2 * np.pi) * tile / 3, (sub, tile)
+
+
 def test_wave_reference_bound_catches_a_wrong_centre():
     """Taking D_k at a corner of the wave block instead of its centre doubles the offset the lanes
     rotate across, and the exact maximum then exceeds the bound (the bound is not vacuous)."""
