@@ -117,11 +117,13 @@ __global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* _
 // HAS_W: lam_p W_cj enters the phase (nf or secondorder, not folded into A).
 // LAM:   secondorder far field unfolded: also accumulate sum_p Im(.) lam_p.
 // AMP:   also accumulate sum_p Re(.) for dL/da.
+// Held to 128 VGPRs (four waves per SIMD; the unfolded secondorder far field with amplitudes
+// would spill, and keeps three).
 template <int EPL, bool HAS_W, bool LAM, bool AMP>
-__global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __restrict__ emit,
-                                                    const double* __restrict__ amp, int64_t n,
-                                                    const float4* __restrict__ pa, const float4* __restrict__ pb,
-                                                    const float2* __restrict__ pc, int tiles_y, int ntiles,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((LAM && AMP) ? 3 : 4)))
+void k_adjoint(Geom g, const double* __restrict__ emit, const double* __restrict__ amp, int64_t n,
+               const float4* __restrict__ pa, const float4* __restrict__ pb, const float2* __restrict__ pc,
+               int tiles_y, int ntiles,
                                                     int tiles_per_split, double* __restrict__ partial)
 {
     constexpr int M = 64 * EPL;  // emitters per workgroup
@@ -163,27 +165,23 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
         __syncthreads();
         if (!active) continue;
 
-        const Frame c = adj_tile_centre(g, t, tiles_y);
-        float rx[EPL], ry[EPL], rz[EPL], rA[EPL], rW[EPL], bx[EPL], by[EPL], bz[EPL], rm[EPL];
+        float rx[EPL], ry[EPL], rz[EPL], rA[EPL], rW[EPL];
+        {
+            const Frame c = adj_tile_centre(g, t, tiles_y);
 #pragma unroll
-        for (int i = 0; i < EPL; ++i) {
-            const int64_t e = ebase + lane + 64 * i;
-            if (e < n) {
-                const double* a4 = emit + 4 * e;
-                const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], 1.0);
-                rx[i] = r.x;
-                ry[i] = r.y;
-                rz[i] = r.z;
-                rA[i] = r.A;
-                rW[i] = r.W;
-                rm[i] = r.amp;  // R_c / D_cj with "scale", else 1
-                // beta_cj in FP32 (|beta| ~ |r_j|/R; only its relative precision matters)
-                const float fx = (float)a4[0], fy = (float)a4[1], fz = (float)a4[2];
-                bx[i] = fx * r.iD - (float)c.rh[0] * r.sigma;
-                by[i] = fy * r.iD - (float)c.rh[1] * r.sigma;
-                bz[i] = fz * r.iD - (float)c.rh[2] * r.sigma;
-            } else {
-                rx[i] = ry[i] = rz[i] = rA[i] = rW[i] = bx[i] = by[i] = bz[i] = rm[i] = 0.f;
+            for (int i = 0; i < EPL; ++i) {
+                const int64_t e = ebase + lane + 64 * i;
+                if (e < n) {
+                    const double* a4 = emit + 4 * e;
+                    const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], 1.0);
+                    rx[i] = r.x;
+                    ry[i] = r.y;
+                    rz[i] = r.z;
+                    rA[i] = r.A;
+                    rW[i] = r.W;
+                } else {
+                    rx[i] = ry[i] = rz[i] = rA[i] = rW[i] = 0.f;
+                }
             }
         }
 
@@ -220,15 +218,34 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(Geom g, const double* __rest
 
         // fold the tile into the FP64 accumulators:
         // sum_p Im_p grad theta_pj = k [(beta - u_c) S + V / krev]
+        // The tile centre and the records are formed again, from a tile index the compiler cannot
+        // equate with t (an empty asm): the values the fold needs (beta, the amplitude factor, u_c)
+        // are then not held in VGPRs across the pixel loop (one FP64 record per emitter and tile,
+        // against 256 pairs).
+        int tf = t;
+        asm volatile("" : "+v"(tf));
+        const Frame c = adj_tile_centre(g, tf, tiles_y);
         const double ik = 1.0 / g.krev;
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
-            const double ma = (double)rm[i];
+            const int64_t e = ebase + lane + 64 * i;
+            float bx = 0.f, by = 0.f, bz = 0.f, rm = 0.f;
+            if (e < n) {
+                const double* a4 = emit + 4 * e;
+                const SplitRec r = make_split_rec(g, c, a4[0], a4[1], a4[2], a4[3], 1.0);
+                rm = r.amp;  // R_c / D_cj with "scale", else 1
+                // beta_cj in FP32 (|beta| ~ |r_j|/R; only its relative precision matters)
+                const float fx = (float)a4[0], fy = (float)a4[1], fz = (float)a4[2];
+                bx = fx * r.iD - (float)c.rh[0] * r.sigma;
+                by = fy * r.iD - (float)c.rh[1] * r.sigma;
+                bz = fz * r.iD - (float)c.rh[2] * r.sigma;
+            }
+            const double ma = (double)rm;
             const double S = ma * (double)Sim[i];
             const double Vfx = ma * (double)Vx[i], Vfy = ma * (double)Vy[i], Vfz = ma * (double)Vz[i];
-            acc[0][i] += g.k * (((double)bx[i] - c.u[0]) * S + Vfx * ik);
-            acc[1][i] += g.k * (((double)by[i] - c.u[1]) * S + Vfy * ik);
-            acc[2][i] += g.k * (((double)bz[i] - c.u[2]) * S + Vfz * ik);
+            acc[0][i] += g.k * (((double)bx - c.u[0]) * S + Vfx * ik);
+            acc[1][i] += g.k * (((double)by - c.u[1]) * S + Vfy * ik);
+            acc[2][i] += g.k * (((double)bz - c.u[2]) * S + Vfz * ik);
             if (!g.nf && g.secondorder) {
                 // theta += k/(2 R_p)(x^2 + y^2) with R_c/R_p = lam (lam ~ 1 when W is folded)
                 const double L = (LAM ? ma * (double)Lm[i] : S) * g.k / c.R;

@@ -1109,15 +1109,17 @@ static int adjoint_call(const char* who, idisim_handle* h, const double* emitter
         int rc = measure_extent(h, emitters, n, st, &rho);
         if (rc) return rc;
     }
-    // the adjoint streams 16 x 16 pixel tiles; it adds the per-pixel phase psi_p (FP32,
-    // |psi| <= 1/2) to the reduced phase: one more rounding at |tau + psi| <= 1 rev
-    const double bound = plan_bound(h, 0, rho) + 0x1p-23;
+    // the adjoint streams 16 x 16 pixel tiles (the planner's tile variant 0 has that shape); it
+    // adds the per-pixel phase psi_p (FP32, |psi| <= 1/2) to the reduced phase: one more rounding
+    // at |tau + psi| <= 1 rev
+    constexpr int adj_tile = 0;
+    const double bound = plan_bound(h, adj_tile, rho) + 0x1p-23;
     if (bound > h->tol_rev)
         return fail(IDISIM_E_PRECISION, std::string(who) + ": FP32-split error bound " + std::to_string(bound) +
                                             " rev exceeds the tolerance for this geometry");
     h->rho = rho;
     bool w = false;
-    const Geom gl = split_geom(h, 0, &w);
+    const Geom gl = split_geom(h, adj_tile, &w);
     const bool lam = w && !gl.nf;
     const AdjointLayout L = adjoint_layout(h->g, n, h->num_cus, adjoint_occupancy(w, lam, grad_amp != nullptr));
     int rc = ensure_scratch(h, L.bytes);
@@ -1126,9 +1128,9 @@ static int adjoint_call(const char* who, idisim_handle* h, const double* emitter
                               target, loss, h->scratch, grad_pos, grad_phase, grad_amp, st));
     const int splits = L.splits;
     h->plan.precision = IDISIM_PREC_FP32SPLIT;
-    h->plan.tile = 0;
-    h->plan.tile_rows = 16;
-    h->plan.tile_cols = 16;
+    h->plan.tile = adj_tile;
+    h->plan.tile_rows = tile_rows(adj_tile);
+    h->plan.tile_cols = tile_cols(adj_tile);
     h->plan.split = splits;
     h->plan.phase_bound = bound;
     h->plan.fallback = 0;

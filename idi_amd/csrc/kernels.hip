@@ -154,24 +154,6 @@ __global__ __launch_bounds__(kBlock) void k_fwd_split(Geom g, const double* __re
 // Items are sorted by emitter count (largest first) on the host, so the tail
 // of the queue is made of short items.  Every workgroup leaves the loop once
 // the counter passes nitems.
-// The Geom argument of a persistent kernel, read again from the kernarg segment for each work item
-// (a kernel's first argument sits at offset 0 of that segment).  Held as a by-value argument, all
-// of it stays live in SGPRs across the emitter loop of every item, and the excess is spilled into
-// VGPR lanes (32 SGPRs and 28 VGPRs in the mirror-pair kernels, one wave per SIMD fewer).  The
-// empty asm makes the pointer opaque, so the loads are not hoisted out of the item loop; they
-// are scalar loads from constant memory.
-__device__ __forceinline__ Geom item_geom()
-{
-    static_assert(sizeof(Geom) % 4 == 0, "Geom is copied as 32-bit words");
-    using KernargWords = const __attribute__((address_space(4))) uint32_t*;
-    KernargWords p = (KernargWords)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(p));
-    Geom g;
-    uint32_t* d = reinterpret_cast<uint32_t*>(&g);
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(Geom) / 4); ++i) d[i] = p[i];
-    return g;
-}
 
 template <int PX, int PY, bool HAS_W, bool HAS_AMP, int SUM>
 __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g0, const double* __restrict__ emit,

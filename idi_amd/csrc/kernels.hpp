@@ -4,6 +4,27 @@
 
 namespace idisim {
 
+// The Geom argument of a kernel, read again from the kernarg segment (a kernel's first argument
+// sits at offset 0 of that segment).  The persistent kernels read it for each work item: held as a
+// by-value argument, all of it stays live in SGPRs across the emitter loop of every item, and the
+// excess is spilled into VGPR lanes (32 SGPRs and 28 VGPRs in the mirror-pair kernels, one wave
+// per SIMD fewer).  A second read after a long loop also keeps the compiler from holding FP64
+// values formed before the loop (pixel frames) in VGPRs across it, when the code after the loop
+// forms the same values again.  The empty asm makes the pointer opaque, so the loads are not
+// hoisted; they are scalar loads from constant memory.
+__device__ __forceinline__ Geom item_geom()
+{
+    static_assert(sizeof(Geom) % 4 == 0, "Geom is copied as 32-bit words");
+    using KernargWords = const __attribute__((address_space(4))) uint32_t*;
+    KernargWords p = (KernargWords)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    Geom g;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&g);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(Geom) / 4); ++i) d[i] = p[i];
+    return g;
+}
+
 constexpr int OUT_FIELD = 0;
 constexpr int OUT_INTENSITY = 1;
 
@@ -40,7 +61,7 @@ int occupancy_batched_fp64(int v);
 // Adjoint (FP32-split) over 16 x 16 pixel tiles.  The scratch buffer holds the per-pixel
 // tile data (float4 + float2 per pixel) followed by double partials [splits][n][5].
 struct AdjointLayout {
-    int tiles_y, ntiles, per, splits;
+    int tiles_y, ntiles, per, splits;  // 16 x 16 tiles along y, in all, per workgroup; splits
     int64_t egroups;
     size_t pix_bytes, part_bytes, bytes;  // pixel tables, split partials, all (with ntiles loss parts)
 };
