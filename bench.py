@@ -471,10 +471,18 @@ def bench_adjoint(args):
     wall, ms = timer.run(step, args.steps)
     h.forward(d, None, field, _lib.OUT_FIELD, ext)
     fwd_plan = h.plan()
+    # the adjoint call alone (pixel table, k_adjoint, finalize) on the last field, for the roofline of
+    # its dominant kernel
+    _, ms_adj = StepTimer().run(lambda: h.adjoint_l2(d, field, target32, None, ext, want_amp=False), args.steps)
+    adj_ms = float(np.mean(ms_adj))
+    roof = roofline_valu(_lib, load_mix("adjoint", 0), n * n0 * n1 / (adj_ms * 1e-3), OPS_PER_PAIR_ADJ,
+                         kernel_ms=adj_ms)
+    roof["kernel_ms_note"] = "the adjoint call: pixel table, k_adjoint and its finalize"
+    log("roofline", roof)
     # the gradients of one step against long-double central differences of the loss (h = 1e-4 nm for
     # positions, 1e-4 rad for phases) on 24 position and 8 phase coordinates, and against the FP64
     # analytic gradient of the oracle over all 4N coordinates
-    check = None
+    check = cpu = None
     if not args.no_cpu:
         import oracle
 
@@ -492,8 +500,14 @@ def bench_adjoint(args):
         fd_s = time.perf_counter() - t0
         gpu = np.array([gpos[j, c] if c < 3 else gph[j] for j, c in sel])
         rms = np.array([np.sqrt(np.mean(gpos**2)) if c < 3 else np.sqrt(np.mean(gph**2)) for _, c in sel])
+        t0 = time.perf_counter()
         fref = oracle.field_c(geom, at, nthreads=nt)
         rpos, rph, _ = oracle.adjoint_c(geom, at, 2 * (np.abs(fref) ** 2 - T), field=fref, nthreads=nt)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": pairs / cpu_s, "unit": "pair-evals/s", "cores": nt, "kind": "port",
+               "sample": f"the whole step: FP64 forward and analytic FP64 adjoint of the oracle (C/OpenMP "
+                         f"restatement of cpu.py:30-107 and its gradient), {pairs:.3e} pairs, "
+                         f"{cpu_s:.2f} s, {cpu_model()}"}
         check = {"fd_coordinates": len(sel), "fd_seconds": fd_s,
                  "loss_rel_err_vs_fp64": abs(loss - float(np.sum((np.abs(fref) ** 2 - T) ** 2))) /
                  float(np.sum((np.abs(fref) ** 2 - T) ** 2)),
@@ -508,7 +522,7 @@ def bench_adjoint(args):
                                    "sum (I - I_target)^2, 256x256 px of 75 um at z=0.1 m, near field; "
                                    "step = forward + adjoint (pairs counted twice)", "pairs_per_step": pairs,
                        "forward_plan": fwd_plan, "plan": h.plan()},
-            "gradient_check": check}
+            "gradient_check": check, "roofline": roof, "cpu_baseline": cpu}
 
 
 def bench_c1(args):
