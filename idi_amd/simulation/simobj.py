@@ -88,21 +88,6 @@ class simobj(_abc.ABC):
             raise ValueError(f"amplitudes must have shape ({self.N},)")
         return a
 
-    def getImg(self, dx, ndim=2):
-        """Emitters binned onto a grid of spacing dx with their phase factors (simobj.py:68-78)."""
-        from ..util import fastlen
-
-        if not 0 < ndim <= 3:
-            raise ValueError
-        pos, phase = self.get2()
-        ind = _np.rint((pos[:, :ndim] - _np.min(pos[:, :ndim], axis=0, keepdims=True)) / dx).astype(int)
-        s = _np.array(ind.max(0)) + 1
-        pads = _np.array([fastlen(i) for i in s])
-        ind += (pads - s) // 2
-        img = _np.zeros(pads, _np.complex128)
-        _np.add.at(img, tuple(ind.T), _np.exp(1j * _np.squeeze(phase, -1)))
-        return img
-
     @_abc.abstractmethod
     def updatePos(self):
         pass
