@@ -231,21 +231,23 @@ __global__ __launch_bounds__(kBlock) void k_batched_split(Geom g0, const double*
 // emitter that is 2 transcendentals and 21 (REF 1) or 27 (REF 2) other VALU where
 // REF 0 spends 6 and 17 (plus 2 multiplies by the amplitude, which with a reference
 // is folded into E_k instead).  The host planner bounds t^2 / 2 (REF 1) or |t|^3 / 6
-// (REF 2).  sD holds the four waves' D_k (FP32, 24 floats) and sE their phasors,
-// (cos, sin) of both pairs per float4, with REF 2 a second float4 scaled by sqrt 2.
+// (REF 2).  sD holds each wave's D_k (FP32, 6 floats of 18 per wave; REF 3 adds its slopes)
+// and sE their phasors, (cos, sin) of both pairs per float4, with REF 2 a second float4
+// scaled by sqrt 2.
 // Emitters staged in LDS per round by the mirror-pair tiles (records and, with a wave reference,
 // the four waves' phasors: 16 B x (1 + 4 or 8) per emitter).
 constexpr int kPairChunk = 256;
 // REF 4 (variant 14) takes a reference for each 8 x 8 pixel sub-block of a wave (four per wave)
 // and rotates to first order; the lanes far from their reference scale their sums at the end
-// (g.refrho, g.refc).  Its 16 phasors per emitter are
-// stored emitter-major with a stride of 17 float4 (a lane's four sub-blocks read adjacent
-// float4s; the record stage's writes from consecutive lanes fall in distinct banks), 128
-// emitters per round, so that the LDS of a workgroup stays under 40 KB (4 per CU).
+// (g.refrho, g.refc).  Its 16 phasors per emitter are stored emitter-major with a stride of
+// 17 float4 (a wave's four sub-blocks read adjacent float4s; the record stage's writes from
+// consecutive lanes fall in distinct banks), 128 emitters per round, so that the LDS of a
+// workgroup stays under 40 KB (4 per CU).
 constexpr int kSubRefs = 16;
 constexpr int kSubStride = 17;
 // REF 3 (variant 13) holds two float4 per emitter and wave: 224 emitters per round keep a
-// workgroup's LDS under 32 KB, so that five workgroups share a CU.
+// workgroup's LDS under 32 KB, so that five single-shot workgroups can share a CU (measured the
+// same as 256 emitters at four).
 constexpr int kRef3Chunk = 224;
 // REF 5 (variant 15) adds one float per emitter and sub-block, the rotation between the pair
 // centres, 2 pi (m_0 - m_1) . r' of the sub-block's reference quad: 112 emitters per round.
