@@ -200,6 +200,8 @@ def bench_single(args):
     hs = _lib.Handle(n0, n1, 75.0, 2e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
     if args.tile is not None:  # pin a tile variant (profiling one kernel); the planner's split
         hs.set_option(_lib.OPT_TILE, args.tile)
+    if args.split is not None:  # pin the emitter split count
+        hs.set_option(_lib.OPT_SPLIT, args.split)
     out = hs.forward(d, None, None, _lib.OUT_FIELD, ext)  # autotune + first run
     torch.cuda.synchronize()
     plan = hs.plan()
@@ -871,6 +873,8 @@ def main():
     ap.add_argument("--tile", type=int, default=None,
                     help="single, batched and large workloads: run this FP32-split tile variant instead of the "
                          "planner's choice")
+    ap.add_argument("--split", type=int, default=None,
+                    help="single workload: run this many emitter splits instead of the planner's choice")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
