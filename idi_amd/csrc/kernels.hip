@@ -283,7 +283,8 @@ __device__ __forceinline__ void pair_lane(int tid, int& lx, int& ly)
     }
 }
 
-template <int QY, bool HAS_AMP, bool CENTRED, int REF>
+// PRIO: the wave priority follows the share of the workgroup's emitters done (prio_by_progress).
+template <int QY, bool HAS_AMP, bool CENTRED, int REF, bool PRIO = false>
 __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restrict__ emit,
                                           const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0,
                                           int ty0, float4* s0, float2* s1, float* sD, float4* sE, float* sC,
@@ -293,6 +294,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
     const int tid = threadIdx.x;
     int lx, ly;
     pair_lane<REF>(tid, lx, ly);
+    if (PRIO) __builtin_amdgcn_s_setprio(3);
 
     double ccx, ccy;
     tile_centre(g, tx0, ty0, TX, TY, ccx, ccy);
@@ -412,6 +414,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
         // of its 16 sub-block phasors
         const int el = SUB ? (tid & 127) : tid;
         __syncthreads();  // the previous chunk has been consumed by every lane
+        if (PRIO) __builtin_amdgcn_s_setprio(3);  // the records (FP64): every wave waits for them
         if (el < cnt) {
             const int64_t e = base + el;
             const double* a4 = emit + 4 * e;
@@ -497,8 +500,13 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             for (int k = 0; k < 2; ++k)
 #pragma unroll
                 for (int m = 0; m < 4; ++m) su[q][k][m] = 0.f;
+        // PRIO: blocks of 32 emitters, each at the priority of the share done before it
+        constexpr int EB = 32;
+        for (int eb = 0; eb < (PRIO ? cnt : 1); eb += EB) {  // one pass without PRIO
+        if (PRIO) prio_by_progress((int)((32 * (base - e0 + eb)) / (e1 - e0)));
+        const int ee = !PRIO ? cnt : (cnt - eb < EB ? cnt : eb + EB);
 #pragma unroll (SUB ? 4 : 2)  // SUB: one address update per four emitters (the VGPR bound holds)
-        for (int e = 0; e < cnt; ++e) {
+        for (int e = eb; e < ee; ++e) {
             const float4 r0 = s0[e];
             const float am = (HAS_AMP && REF == 0) ? s1[e].y : 1.f;
             float4 E = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E;
@@ -560,6 +568,7 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
                     su[q][k][3] = fmaf(ps[k], sd, su[q][k][3]);
                 }
             }
+        }
         }
 #pragma unroll
         for (int q = 0; q < QY; ++q)
@@ -762,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
     const int64_t e0 = (int64_t)split * per_split;
     const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
     float acc[QY][2][4], cmp[QY][2][4];
-    pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, sD, sE, sC, acc, cmp);
+    pair_tile<QY, HAS_AMP, CENTRED, REF, true>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, sD, sE, sC, acc, cmp);
 
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);
