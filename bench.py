@@ -142,27 +142,34 @@ def cpu_model():
 
 
 class StepTimer:
-    """HIP events around each step on torch's current stream (the library's launch stream)."""
+    """Wall time of `steps` calls between two synchronisations and, with per_step, HIP events around
+    each call on torch's current stream (the library's launch stream).  A pair of event records
+    between two steps leaves the GPU idle for about 10 us before the next step's first kernel
+    (measured in a kernel trace of the 0.15 ms adjoint step), so the short steps take their wall
+    time from a run without them."""
 
     def __init__(self):
         import torch
 
         self.torch = torch
-        self.ev = []
 
-    def run(self, fn, steps):
+    def run(self, fn, steps, per_step=True):
         torch = self.torch
-        pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                 for _ in range(steps if per_step else 0)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for a, b in pairs:
-            a.record()
-            fn()
-            b.record()
+        if per_step:
+            for a, b in pairs:
+                a.record()
+                fn()
+                b.record()
+        else:
+            for _ in range(steps):
+                fn()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        ms = [a.elapsed_time(b) for a, b in pairs]
-        return wall, ms
+        return wall, [a.elapsed_time(b) for a, b in pairs]
 
 
 def cpu_baseline_rows(oracle, geom, atoms, amp, budget_s, nthreads):
@@ -208,7 +215,8 @@ def bench_single(args):
     log("fp32-split plan", plan)
     timer = StepTimer()
     timer.run(lambda: hs.forward(d, None, out, _lib.OUT_FIELD, ext), args.warmup)
-    wall, ms = timer.run(lambda: hs.forward(d, None, out, _lib.OUT_FIELD, ext), args.steps)
+    wall, _ = timer.run(lambda: hs.forward(d, None, out, _lib.OUT_FIELD, ext), args.steps, per_step=False)
+    _, ms = timer.run(lambda: hs.forward(d, None, out, _lib.OUT_FIELD, ext), args.steps)
     split_ms = float(np.mean(ms))
     value = pairs * args.steps / wall
     field_split = out.cpu().numpy()
@@ -469,8 +477,8 @@ def bench_adjoint(args):
     step()
     torch.cuda.synchronize()
     timer = StepTimer()
-    timer.run(step, args.warmup)
-    wall, ms = timer.run(step, args.steps)
+    timer.run(step, args.warmup, per_step=False)
+    wall, _ = timer.run(step, args.steps, per_step=False)
     h.forward(d, None, field, _lib.OUT_FIELD, ext)
     fwd_plan = h.plan()
     # the adjoint call alone (pixel table, k_adjoint, finalize) on the last field, for the roofline of

@@ -245,10 +245,11 @@ constexpr int kPairChunk = 256;
 // workgroup stays under 40 KB (4 per CU).
 constexpr int kSubRefs = 16;
 constexpr int kSubStride = 17;
-// REF 3 (variant 13) holds two float4 per emitter and wave: 224 emitters per round keep a
-// workgroup's LDS under 32 KB, so that five single-shot workgroups can share a CU (measured the
-// same as 256 emitters at four).
-constexpr int kRef3Chunk = 224;
+// REF 3 (variant 13) holds two float4 per emitter and wave: 256 emitters per round take 36 KB of
+// LDS, four workgroups per CU.  224 (five per CU under 32 KB) measured the same on 100k emitters,
+// but a 256-emitter split then takes two rounds, the second of 32 emitters (4096 emitters over
+// 16 splits: 5 us of a 51 us launch).
+constexpr int kRef3Chunk = 256;
 // REF 5 (variant 15) adds one float per emitter and sub-block, the rotation between the pair
 // centres, 2 pi (m_0 - m_1) . r' of the sub-block's reference quad: 112 emitters per round.
 constexpr int kRef5Chunk = 112;
@@ -283,33 +284,28 @@ __device__ __forceinline__ void pair_lane(int tid, int& lx, int& ly)
     }
 }
 
-// PRIO: the wave priority follows the share of the workgroup's emitters done (prio_by_progress).
-template <int QY, bool HAS_AMP, bool CENTRED, int REF, bool PRIO = false>
-__device__ __forceinline__ void pair_tile(const Geom& g, const double* __restrict__ emit,
-                                          const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0,
-                                          int ty0, float4* s0, float2* s1, float* sD, float4* sE, float* sC,
-                                          float (&acc)[QY][2][4], float (&cmp)[QY][2][4])
+// Set-up of a mirror-pair tile from the detector geometry alone: the tile-centre frame c, each
+// lane's quad terms mc, dv and d (d relative to the lane's reference), and the workgroup's
+// reference data in sD (REF >= 1).  REF 3 stages its half differences in sE and synchronises the
+// workgroup.  About nine FP64 pixel frames per lane: k_pair_table forms them once per handle and
+// tile variant, and k_fwd_pair then loads them (pair_tab_floats).
+template <int QY, bool CENTRED, int REF>
+__device__ __forceinline__ void pair_setup(const Geom& g, int tx0, int ty0, int tid, int lx, int ly, int myref,
+                                           float* sD, float4* sE, Frame& c, float (&mc)[QY][3],
+                                           float (&dv)[QY][3], float (&d)[QY][2][3])
 {
     constexpr int TX = 32, TY = 32 * QY;
-    const int tid = threadIdx.x;
-    int lx, ly;
-    pair_lane<REF>(tid, lx, ly);
-    if (PRIO) __builtin_amdgcn_s_setprio(3);
-
     double ccx, ccy;
     tile_centre(g, tx0, ty0, TX, TY, ccx, ccy);
-    const Frame c = pixel_frame(g, ccx, ccy);
+    c = pixel_frame(g, ccx, ccy);
 
     // the wave's reference half differences D_k (as the FP32 values the records use)
     double Dref[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-    // REF 3: the quad centre's offset from the block centre (pixels), and the wave's linear
-    // reference (D_k and its slopes, 18 floats per wave) formed by one lane of the wave
-    float ox = 0.f, oy = 0.f;
+    // REF 3: the wave's linear reference (D_k and its slopes, 18 floats per wave) formed by one
+    // lane of the wave
     if (REF == 3) {
         static_assert(REF != 3 || QY == 1, "the linear wave reference has one quad per lane");
         const int w = tid >> 6, l = tid & 63;
-        ox = (float)(2 * (l >> 3) - 7);
-        oy = (float)(2 * (l & 7) - 7);
         // lanes 0..4 of each wave form the half differences of the quads centred at the block
         // centre and one pixel to either side (the terms of wave_linear_ref), staged in the
         // phasor area of LDS, which no lane reads before the first chunk's records
@@ -336,12 +332,10 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
     }
     constexpr bool SUB = REF == 4 || REF == 5;  // a reference per 8 x 8 pixel sub-block
     constexpr int DSTR = REF == 5 ? 9 : 6;       // floats per sub-block in sD
-    int myref = 0;  // REF 4, 5: the lane's sub-block, 4 w + (2 x half + y half of the wave block)
     if (REF == 1 || REF == 2 || SUB) {
         static_assert(!SUB || (QY == 1 && CENTRED), "sub-block references: one centred quad per lane");
         const int w = tid >> 6, l = tid & 63;
-        const int sb = ((l >> 5) << 1) | ((l >> 2) & 1);
-        myref = 4 * w + sb;
+        const int sb = myref & 3;
         // the reference quad's centre: the wave block's centre (REF 1, 2) or the sub-block's
         const double px = SUB ? (double)(tx0 + 16 * (w >> 1) + 8 * (sb >> 1)) + 3.5
                               : (double)(tx0 + 16 * (w >> 1)) + 7.5;
@@ -370,7 +364,6 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
     // REF 1, 4: t in radians; REF 2: t / sqrt 2, so that its square is t^2 / 2
     const double dscale = REF == 2 ? kTwoPi * 0.70710678118654752440 : kTwoPi;
 
-    float mc[QY][3], d[QY][2][3], dv[QY][3];
 #pragma unroll
     for (int q = 0; q < QY; ++q) {
         double dq[2][2][3];
@@ -394,6 +387,72 @@ __device__ __forceinline__ void pair_tile(const Geom& g, const double* __restric
             d[q][0][a] = (float)(REF ? dscale * (h0 - Dref[0][a]) : h0);
             d[q][1][a] = (float)(REF ? dscale * (h1 - Dref[1][a]) : h1);
         }
+    }
+}
+
+// The lane's sub-block (REF 4, 5): 4 w + (2 x half + y half of the wave's 16 x 16 pixel block).
+__device__ __forceinline__ int pair_subblock(int tid)
+{
+    const int w = tid >> 6, l = tid & 63;
+    return 4 * w + (((l >> 5) << 1) | ((l >> 2) & 1));
+}
+
+// Table of pair_setup for one tile (one quad per lane), in floats: the lanes' mc, dv, d as three
+// float4 planes of kBlock, the reference data sD (padded to whole float4), and the tile-centre
+// frame as 8 doubles (R, rh, u, R - z).
+constexpr int kPairTabLane = 3 * 4 * kBlock;
+__host__ __device__ constexpr int pair_tab_sd(int ref) { return ref == 0 ? 0 : (pair_sd_floats(ref) + 3) / 4 * 4; }
+__host__ __device__ constexpr int pair_tab_floats(int ref) { return kPairTabLane + pair_tab_sd(ref) + 16; }
+
+// PRIO: the wave priority follows the share of the workgroup's emitters done (prio_by_progress).
+// tab: this tile's pair_setup table (QY 1), or nullptr to form it here.
+template <int QY, bool HAS_AMP, bool CENTRED, int REF, bool PRIO = false>
+__device__ __forceinline__ void pair_tile(const Geom& g, const double* __restrict__ emit,
+                                          const double* __restrict__ amp, int64_t e0, int64_t e1, int tx0,
+                                          int ty0, const float* __restrict__ tab, float4* s0, float2* s1,
+                                          float* sD, float4* sE, float* sC, float (&acc)[QY][2][4],
+                                          float (&cmp)[QY][2][4])
+{
+    const int tid = threadIdx.x;
+    int lx, ly;
+    pair_lane<REF>(tid, lx, ly);
+    if (PRIO) __builtin_amdgcn_s_setprio(3);
+
+    // REF 3: the quad centre's offset from the wave block's centre (pixels)
+    float ox = 0.f, oy = 0.f;
+    if (REF == 3) {
+        const int l = tid & 63;
+        ox = (float)(2 * (l >> 3) - 7);
+        oy = (float)(2 * (l & 7) - 7);
+    }
+    constexpr bool SUB = REF == 4 || REF == 5;  // a reference per 8 x 8 pixel sub-block
+    const int myref = SUB ? pair_subblock(tid) : 0;
+
+    Frame c;
+    float mc[QY][3], d[QY][2][3], dv[QY][3];
+    bool loaded = false;
+    if constexpr (QY == 1) {
+        if (tab != nullptr) {
+            const float4* tl = reinterpret_cast<const float4*>(tab);
+            const float4 t0 = tl[tid], t1 = tl[kBlock + tid], t2 = tl[2 * kBlock + tid];
+            mc[0][0] = t0.x, mc[0][1] = t0.y, mc[0][2] = t0.z, dv[0][0] = t0.w;
+            dv[0][1] = t1.x, dv[0][2] = t1.y, d[0][0][0] = t1.z, d[0][0][1] = t1.w;
+            d[0][0][2] = t2.x, d[0][1][0] = t2.y, d[0][1][1] = t2.z, d[0][1][2] = t2.w;
+            // sD is read after the first chunk's barrier
+            for (int i = tid; i < pair_tab_sd(REF); i += kBlock)
+                if (i < pair_sd_floats(REF)) sD[i] = tab[kPairTabLane + i];
+            const double* fr = reinterpret_cast<const double*>(tab + kPairTabLane + pair_tab_sd(REF));
+            c.R = fr[0];
+            c.rh[0] = fr[1], c.rh[1] = fr[2], c.rh[2] = fr[3];
+            c.u[0] = fr[4], c.u[1] = fr[5], c.u[2] = fr[6];
+            c.RmZ = fr[7];
+            loaded = true;
+        }
+    }
+    if (!loaded) pair_setup<QY, CENTRED, REF>(g, tx0, ty0, tid, lx, ly, myref, sD, sE, c, mc, dv, d);
+    constexpr int DSTR = REF == 5 ? 9 : 6;  // floats per sub-block in sD
+#pragma unroll
+    for (int q = 0; q < QY; ++q) {
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
 #pragma unroll
@@ -753,11 +812,12 @@ __device__ __forceinline__ void split_hi_lo(double v, float& hi, float& lo)
     lo = (float)((double)hi - v);
 }
 
+// tab: the tiles' pair_setup tables (k_pair_table; QY 1), or nullptr to form them in the prologue.
 template <int QY, bool HAS_AMP, bool CENTRED, int REF>
 __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __restrict__ emit,
                                                      const double* __restrict__ amp, int64_t n, int tiles_y,
-                                                     int64_t per_split, float4* __restrict__ partial, void* out,
-                                                     int kind)
+                                                     int64_t per_split, const float* __restrict__ tab,
+                                                     float4* __restrict__ partial, void* out, int kind)
 {
     constexpr int TX = 32, TY = 32 * QY;
     __shared__ float4 s0[pair_chunk(REF)];
@@ -771,7 +831,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
     const int64_t e0 = (int64_t)split * per_split;
     const int64_t e1 = (e0 + per_split < n) ? e0 + per_split : n;
     float acc[QY][2][4], cmp[QY][2][4];
-    pair_tile<QY, HAS_AMP, CENTRED, REF, true>(g, emit, amp, e0, e1, tx0, ty0, s0, s1, sD, sE, sC, acc, cmp);
+    const float* tt = tab != nullptr ? tab + (size_t)tile * pair_tab_floats(REF) : nullptr;
+    pair_tile<QY, HAS_AMP, CENTRED, REF, true>(g, emit, amp, e0, e1, tx0, ty0, tt, s0, s1, sD, sE, sC, acc, cmp);
 
     int lx, ly;
     pair_lane<REF>(threadIdx.x, lx, ly);
@@ -799,6 +860,38 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pair(Geom g, const double* __res
                 }
             }
         }
+    }
+}
+
+// The pair_setup table of every 32 x 32 pixel tile (one quad per lane): one workgroup per tile.
+template <bool CENTRED, int REF>
+__global__ __launch_bounds__(kBlock) void k_pair_table(Geom g, int tiles_y, float* __restrict__ tab)
+{
+    __shared__ float sD[pair_sd_floats(REF)];
+    __shared__ float4 sE[REF == 3 ? 60 : 1];  // REF 3: the half differences, 4 waves x 5 lanes x 6 doubles
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int tx0 = (tile / tiles_y) * 32, ty0 = (tile % tiles_y) * 32;
+    for (int i = tid; i < pair_sd_floats(REF); i += kBlock) sD[i] = 0.f;
+    __syncthreads();
+    int lx, ly;
+    pair_lane<REF>(tid, lx, ly);
+    const int myref = (REF == 4 || REF == 5) ? pair_subblock(tid) : 0;
+    Frame c;
+    float mc[1][3], dv[1][3], d[1][2][3];
+    pair_setup<1, CENTRED, REF>(g, tx0, ty0, tid, lx, ly, myref, sD, sE, c, mc, dv, d);
+    float* t = tab + (size_t)tile * pair_tab_floats(REF);
+    float4* tl = reinterpret_cast<float4*>(t);
+    tl[tid] = make_float4(mc[0][0], mc[0][1], mc[0][2], dv[0][0]);
+    tl[kBlock + tid] = make_float4(dv[0][1], dv[0][2], d[0][0][0], d[0][0][1]);
+    tl[2 * kBlock + tid] = make_float4(d[0][0][2], d[0][1][0], d[0][1][1], d[0][1][2]);
+    __syncthreads();  // sD complete
+    for (int i = tid; i < pair_tab_sd(REF); i += kBlock) t[kPairTabLane + i] = i < pair_sd_floats(REF) ? sD[i] : 0.f;
+    if (tid == 0) {
+        double* fr = reinterpret_cast<double*>(t + kPairTabLane + pair_tab_sd(REF));
+        fr[0] = c.R;
+        fr[1] = c.rh[0], fr[2] = c.rh[1], fr[3] = c.rh[2];
+        fr[4] = c.u[0], fr[5] = c.u[1], fr[6] = c.u[2];
+        fr[7] = c.RmZ;
     }
 }
 
@@ -913,7 +1006,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QY == 1 
         const Geom g = item_geom();
         const int tx0 = (w.tile / tiles_y) * TX, ty0 = (w.tile % tiles_y) * TY;
         float acc[QY][2][4], cmp[QY][2][4];
-        pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, w.e0, w.e1, tx0, ty0, s0, s1, sD, sE, sC, acc, cmp);
+        pair_tile<QY, HAS_AMP, CENTRED, REF>(g, emit, amp, w.e0, w.e1, tx0, ty0, nullptr, s0, s1, sD, sE, sC, acc,
+                                             cmp);
         char* outs = static_cast<char*>(out) +
                      (size_t)w.shot * npix * (kind == OUT_INTENSITY ? sizeof(float) : sizeof(float2));
         // read again, so that the pixel frames of the stores are formed here and not kept in
@@ -1276,14 +1370,23 @@ int occupancy_batched_split(int v, bool w, bool a, int s)
 
 template <int QY, bool A, bool C, int R>
 static hipError_t launch_pair_q(const Geom& g, const double* emit, const double* amp, int64_t n, int splits,
-                                float4* partial, void* out, int kind, hipStream_t st)
+                                const float* tab, float4* partial, void* out, int kind, hipStream_t st)
 {
     const int tiles_x = (g.n0 + 31) / 32;
     const int tiles_y = (g.n1 + 32 * QY - 1) / (32 * QY);
     const int64_t per = split_size(n, splits);
     dim3 grid(tiles_x * tiles_y, splits);
     hipLaunchKernelGGL((k_fwd_pair<QY, A, C, R>), grid, dim3(kBlock), 0, st, g, emit, amp, n, tiles_y,
-                       per, partial, out, kind);
+                       per, QY == 1 ? tab : nullptr, partial, out, kind);
+    return hipGetLastError();
+}
+
+template <int QY, bool A, bool C, int R>
+static hipError_t launch_pair_table_q(const Geom& g, float* tab, hipStream_t st)
+{
+    if (QY != 1) return hipErrorInvalidValue;
+    const int tiles_x = (g.n0 + 31) / 32, tiles_y = (g.n1 + 31) / 32;
+    hipLaunchKernelGGL((k_pair_table<C, R>), dim3(tiles_x * tiles_y), dim3(kBlock), 0, st, g, tiles_y, tab);
     return hipGetLastError();
 }
 
@@ -1369,13 +1472,28 @@ static int occupancy_block_a()
 }
 
 hipError_t launch_fwd_pair(int v, bool a, const Geom& g, const double* emit, const double* amp, int64_t n,
-                           int splits, float4* partial, void* out, int kind, hipStream_t st)
+                           int splits, const float* tab, float4* partial, void* out, int kind, hipStream_t st)
 {
     if (g.foldw == 0 && (g.nf || g.secondorder)) return hipErrorInvalidValue;  // needs W folded into A
     if (v == kTileBlock)
         return a ? launch_block_a<true>(g, emit, amp, n, splits, partial, out, kind, st)
                  : launch_block_a<false>(g, emit, amp, n, splits, partial, out, kind, st);
-    IDISIM_DISPATCH_PAIR(launch_pair_q, g, emit, amp, n, splits, partial, out, kind, st)
+    IDISIM_DISPATCH_PAIR(launch_pair_q, g, emit, amp, n, splits, tab, partial, out, kind, st)
+    return hipErrorInvalidValue;
+}
+
+size_t pair_table_bytes(int v, const Geom& g)
+{
+    if (!tile_pair(v) || v == kTileBlock || pair_quads(v) != 1) return 0;
+    const size_t tiles = (size_t)((g.n0 + 31) / 32) * ((g.n1 + 31) / 32);
+    return tiles * pair_tab_floats(tile_ref_kind(v)) * sizeof(float);
+}
+
+hipError_t launch_pair_table(int v, const Geom& g, float* tab, hipStream_t st)
+{
+    if (pair_table_bytes(v, g) == 0) return hipErrorInvalidValue;
+    const bool a = false;  // the table does not depend on the amplitudes
+    IDISIM_DISPATCH_PAIR(launch_pair_table_q, g, tab, st)
     return hipErrorInvalidValue;
 }
 

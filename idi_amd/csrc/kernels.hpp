@@ -25,17 +25,16 @@ __device__ __forceinline__ Geom item_geom()
     return g;
 }
 
-// Wave priority from the share of its work that a wave has done, q of 32 (s_setprio: 3 first).
-// Waves of different workgroups that share a SIMD otherwise issue oldest first: the oldest
-// workgroup runs ahead, and the last one finishes alone on its SIMD at a fraction of the issue
-// rate (measured at 4096 emitters: the four workgroups of a CU ended 36, 47, 63 and 78 us into
-// the adjoint).  With the waves that are behind issuing first, they finish together.  The
-// thresholds give the last 1/16 of the work its own level.
+// Wave priority of a pixel or emitter loop from the share of its work that a wave has done, q of
+// 32 (s_setprio; the kernels run their FP64 record stages at 3, above every loop).  Waves of
+// different workgroups that share a SIMD otherwise issue oldest first: the oldest workgroup runs
+// ahead, and the last one finishes alone on its SIMD at a fraction of the issue rate (measured at
+// 4096 emitters: the four workgroups of a CU ended 36, 47, 63 and 78 us into the adjoint).  With
+// the waves that are behind issuing first, they finish together; the last 1/16 of the work has a
+// level of its own.
 __device__ __forceinline__ void prio_by_progress(int q)
 {
-    if (q < 16)
-        __builtin_amdgcn_s_setprio(3);
-    else if (q < 26)
+    if (q < 26)
         __builtin_amdgcn_s_setprio(2);
     else if (q < 30)
         __builtin_amdgcn_s_setprio(1);
@@ -58,9 +57,14 @@ hipError_t launch_batched_split(int v, bool w, bool a, int s, const Geom& g, con
                                 void* out, int kind, hipStream_t st);
 int occupancy_batched_split(int v, bool w, bool a, int s);
 // Mirror-pair tiles (tile_pair(v)); W must be folded into A (g.foldw) when the
-// phase model has a curvature / second-order term.
+// phase model has a curvature / second-order term.  tab: the variant's set-up table
+// (launch_pair_table) or nullptr, when the kernel forms it in its prologue.
 hipError_t launch_fwd_pair(int v, bool a, const Geom& g, const double* emit, const double* amp, int64_t n,
-                           int splits, float4* partial, void* out, int kind, hipStream_t st);
+                           int splits, const float* tab, float4* partial, void* out, int kind, hipStream_t st);
+// The set-up table of a mirror-pair variant with one quad per lane (0 bytes: the variant has none):
+// the per-lane quad terms, reference data and tile-centre frame, from the geometry alone.
+size_t pair_table_bytes(int v, const Geom& g);
+hipError_t launch_pair_table(int v, const Geom& g, float* tab, hipStream_t st);
 hipError_t launch_batched_pair(int v, bool a, const Geom& g, const double* emit, const double* amp,
                                const WorkItem* items, int nitems, int* counter, int nblocks, void* out, int kind,
                                hipStream_t st);
