@@ -460,6 +460,10 @@ def bench_adjoint(args):
     at = at0.copy()
     at[:, :3] += rng.normal(0, 5e-5, (n, 3))
     h = _lib.Handle(n0, n1, 75.0, 1e5, K, nf=True, precision=_lib.PREC_FP32SPLIT)
+    if args.tile is not None:  # pin the forward's tile variant
+        h.set_option(_lib.OPT_TILE, args.tile)
+    if args.split is not None:
+        h.set_option(_lib.OPT_SPLIT, args.split)
     target = h.forward(torch.from_numpy(at0).cuda(), None, None, _lib.OUT_INTENSITY).double()
     d = torch.from_numpy(at).cuda()
     ext = float(np.linalg.norm(at[:, :3], axis=1).max())
@@ -879,10 +883,10 @@ def main():
                                                          "correlate", "generate", "issue"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile", type=int, default=None,
-                    help="single, batched and large workloads: run this FP32-split tile variant instead of the "
-                         "planner's choice")
+                    help="single, batched, large and adjoint (its forward) workloads: run this FP32-split tile "
+                         "variant instead of the planner's choice")
     ap.add_argument("--split", type=int, default=None,
-                    help="single workload: run this many emitter splits instead of the planner's choice")
+                    help="single and adjoint workloads: run this many emitter splits instead of the planner's choice")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--traffic", type=float, default=None, help="HBM bytes per launch from rocprofv3 --pmc")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
