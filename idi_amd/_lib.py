@@ -93,6 +93,7 @@ def load():
     lib.idisim_forward_batched.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_int64), i64, dbl, vp, ip, vp]
     lib.idisim_adjoint.argtypes = [vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp, vp]
     lib.idisim_adjoint_l2.argtypes = [vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp, vp, vp]
+    lib.idisim_loss_grad_l2.argtypes = [vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp, vp, vp]
     lib.idisim_extent.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), vp]
     lib.idisim_generate_ball.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, dbl,
                                          ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
@@ -108,7 +109,7 @@ def load():
     lib.idisim_generate_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ip, ip, vp, vp]
     for name in ("idisim_create", "idisim_destroy", "idisim_set_option", "idisim_get_plan", "idisim_forward",
-                 "idisim_forward_noalloc", "idisim_adjoint_l2",
+                 "idisim_forward_noalloc", "idisim_adjoint_l2", "idisim_loss_grad_l2",
                  "idisim_forward_batched", "idisim_adjoint", "idisim_extent", "idisim_generate_ball",
                  "idisim_chargesharing", "idisim_generate_balls", "idisim_generate_gauss",
                  "idisim_corr_accumulate", "idisim_qcorr", "idisim_time_forward", "idisim_generate_choice",
@@ -470,6 +471,33 @@ class Handle:
         gamp = torch.empty((n,), dtype=torch.float32, device=emit.device) if want_amp else None
         check(self._lib.idisim_adjoint_l2(self._h, _ptr(emit), _ptr(amp), int(n), float(extent), _ptr(field),
                                           _ptr(target), _ptr(loss), _ptr(gpos), _ptr(gph), _ptr(gamp), _stream()))
+        return loss, gpos, gph, gamp
+
+    def loss_grad_l2(self, emit, target, amp=None, extent=0.0, field=None, want_amp=True):
+        """forward(emit, amp, field, OUT_FIELD) and adjoint_l2 on that field in one call, the same
+        results bit for bit (idisim_loss_grad_l2): returns (L as a 0-d float64 CUDA tensor, pos (N, 3),
+        phase (N,), amp (N,) or None); `field` (complex64 (n0, n1)), when given, receives the field."""
+        import torch
+
+        n = emit.shape[0]
+        if n < 1:
+            raise ValueError("the loss needs at least one emitter")
+        if emit.dtype != torch.float64 or emit.dim() != 2 or emit.shape[1] != 4 or not emit.is_contiguous():
+            raise ValueError("emitters must be a contiguous (N, 4) float64 tensor")
+        if amp is not None and (amp.dtype != torch.float64 or amp.shape != (n,) or not amp.is_contiguous()):
+            raise ValueError("amplitudes must be a contiguous (N,) float64 tensor")
+        if target.dtype != torch.float32 or tuple(target.shape) != self.shape or not target.is_contiguous():
+            raise ValueError("target must be a contiguous float32 (n0, n1) tensor")
+        if field is not None and (field.dtype != torch.complex64 or tuple(field.shape) != self.shape or
+                                  not field.is_contiguous()):
+            raise ValueError("field must be a contiguous complex64 (n0, n1) tensor")
+        self._on_device(emitters=emit, amplitudes=amp, target=target, field=field)
+        loss = torch.empty((), dtype=torch.float64, device=emit.device)
+        gpos = torch.empty((n, 3), dtype=torch.float32, device=emit.device)
+        gph = torch.empty((n,), dtype=torch.float32, device=emit.device)
+        gamp = torch.empty((n,), dtype=torch.float32, device=emit.device) if want_amp else None
+        check(self._lib.idisim_loss_grad_l2(self._h, _ptr(emit), _ptr(amp), int(n), float(extent), _ptr(target),
+                                            _ptr(field), _ptr(loss), _ptr(gpos), _ptr(gph), _ptr(gamp), _stream()))
         return loss, gpos, gph, gamp
 
     def time_forward(self, emit, times, c, tau, out=None):

@@ -33,7 +33,7 @@
 //                 FP64 sums are reduced through LDS and written as partials [splits][n][5].
 //   k_adjoint_finalize  sums the partials over splits in FP64.
 // Four emitters per lane measured faster than two or eight at 4096 emitters on 256^2 pixels.
-#include "kernels.hpp"
+#include "device_util.hpp"
 
 namespace idisim {
 
@@ -50,8 +50,65 @@ __device__ inline Frame adj_tile_centre(const Geom& g, int t, int tiles_y)
 }
 }  // namespace
 
+// The pixel-table entries of pixel (x, y) of 16 x 16 tile t, from its complex64 field value E,
+// its frame f and (nf) its pixel_phasor (cs, sn).
 // target == nullptr: G_p = dLdI[p].  Otherwise the squared-intensity loss L = sum_p (I_p - T_p)^2:
-// G_p = 2 (I_p - T_p) with I_p = |E_p|^2 in FP64, and the tile's part of L goes to loss_tile[t].
+// G_p = 2 (I_p - T_p) with I_p = |E_p|^2 in FP64, and res2 = (I_p - T_p)^2.
+__device__ inline void adj_pixel_terms(const Geom& g, int t, int tiles_y, int x, int y, const Frame& f, double cs,
+                                       double sn, float2 E, const float* __restrict__ dLdI,
+                                       const float* __restrict__ target, float4& a, float4& b, float2& c2,
+                                       double& res2)
+{
+    const Frame c = adj_tile_centre(g, t, tiles_y);
+    const double dqx = -g.krev * (f.u[0] - c.u[0]), dqy = -g.krev * (f.u[1] - c.u[1]);
+    const double dqz = -g.krev * (f.u[2] - c.u[2]), lam = c.R / f.R;
+    const size_t idx = (size_t)x * g.n1 + y;
+    double G;
+    if (target != nullptr) {
+        const double res = (double)E.x * (double)E.x + (double)E.y * (double)E.y - (double)target[idx];
+        res2 = res * res;
+        G = 2.0 * res;
+    } else {
+        G = (double)dLdI[idx];
+    }
+    double wr = G * (double)E.x, wi = -G * (double)E.y;
+    if (g.nf) {
+        const double r2 = wr * cs - wi * sn;
+        wi = wr * sn + wi * cs;
+        wr = r2;
+    }
+    if (g.scale) {  // the per-pixel factor 1/R_p of the "scale" amplitude R_c/(D_cj R_p)
+        wr /= f.R;
+        wi /= f.R;
+    }
+    const double m = sqrt(wr * wr + wi * wi);
+    const double psi = (m > 0.0) ? atan2(wi, wr) * kInv2Pi : 0.0;
+    a = make_float4((float)dqx, (float)dqy, (float)dqz, (float)psi);
+    b = make_float4((float)(m * dqx), (float)(m * dqy), (float)(m * dqz), (float)m);
+    c2 = make_float2((float)lam, (float)(m * lam));
+}
+
+// Stores the thread's table entries and, with a target, the tile's sum of squared residuals
+// (an LDS tree in a fixed order) into loss_tile[t].
+__device__ inline void adj_pixel_store(int t, int l, float4 a, float4 b, float2 c2, double res2, bool loss,
+                                       float4* __restrict__ pa, float4* __restrict__ pb, float2* __restrict__ pc,
+                                       double* __restrict__ loss_tile, double* red)
+{
+    pa[(size_t)t * kAdjTile + l] = a;
+    pb[(size_t)t * kAdjTile + l] = b;
+    pc[(size_t)t * kAdjTile + l] = c2;
+    if (loss) {
+        red[l] = res2;
+        __syncthreads();
+        for (int s = kAdjTile / 2; s > 0; s >>= 1) {
+            if (l < s) red[l] += red[l + s];
+            __syncthreads();
+        }
+        if (l == 0) loss_tile[t] = red[0];
+    }
+}
+
+// The pixel table from the field of a forward call (one workgroup per 16 x 16 tile).
 __global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* __restrict__ field,
                                                          const float* __restrict__ dLdI,
                                                          const float* __restrict__ target, int tiles_y,
@@ -66,52 +123,44 @@ __global__ __launch_bounds__(kAdjTile) void k_adj_pixels(Geom g, const float2* _
     float2 c2 = make_float2(1.f, 0.f);
     double res2 = 0.0;
     if (x < g.n0 && y < g.n1) {
-        const Frame c = adj_tile_centre(g, t, tiles_y);
         const Frame f = pixel_frame(g, (double)x, (double)y);
-        const double dqx = -g.krev * (f.u[0] - c.u[0]), dqy = -g.krev * (f.u[1] - c.u[1]);
-        const double dqz = -g.krev * (f.u[2] - c.u[2]), lam = c.R / f.R;
-        const size_t idx = (size_t)x * g.n1 + y;
-        const float2 E = field[idx];
-        double G;
-        if (target != nullptr) {
-            const double res = (double)E.x * (double)E.x + (double)E.y * (double)E.y - (double)target[idx];
-            res2 = res * res;
-            G = 2.0 * res;
-        } else {
-            G = (double)dLdI[idx];
-        }
-        double wr = G * (double)E.x, wi = -G * (double)E.y;
-        if (g.nf) {
-            double ph = g.krev * f.RmZ;
-            ph -= rint(ph);
-            double sn, cs;
-            sincos(kTwoPi * ph, &sn, &cs);
-            const double r2 = wr * cs - wi * sn;
-            wi = wr * sn + wi * cs;
-            wr = r2;
-        }
-        if (g.scale) {  // the per-pixel factor 1/R_p of the "scale" amplitude R_c/(D_cj R_p)
-            wr /= f.R;
-            wi /= f.R;
-        }
-        const double m = sqrt(wr * wr + wi * wi);
-        const double psi = (m > 0.0) ? atan2(wi, wr) * kInv2Pi : 0.0;
-        a = make_float4((float)dqx, (float)dqy, (float)dqz, (float)psi);
-        b = make_float4((float)(m * dqx), (float)(m * dqy), (float)(m * dqz), (float)m);
-        c2 = make_float2((float)lam, (float)(m * lam));
+        double cs = 1.0, sn = 0.0;
+        if (g.nf) pixel_phasor(g, f, cs, sn);
+        adj_pixel_terms(g, t, tiles_y, x, y, f, cs, sn, field[(size_t)x * g.n1 + y], dLdI, target, a, b, c2, res2);
     }
-    pa[(size_t)t * kAdjTile + l] = a;
-    pb[(size_t)t * kAdjTile + l] = b;
-    pc[(size_t)t * kAdjTile + l] = c2;
-    if (target != nullptr) {  // the tile's sum of squared residuals, in a fixed order
-        red[l] = res2;
-        __syncthreads();
-        for (int s = kAdjTile / 2; s > 0; s >>= 1) {
-            if (l < s) red[l] += red[l + s];
-            __syncthreads();
-        }
-        if (l == 0) loss_tile[t] = red[0];
+    adj_pixel_store(t, l, a, b, c2, res2, target != nullptr, pa, pb, pc, loss_tile, red);
+}
+
+// The same table straight from a forward launch's split partials (float4 [splits][npix], the
+// mirror-pair kernels' form): each pixel's field is summed and formed as k_finalize_split and
+// split_field do (the same bits), stored when field != nullptr, and used at once.  One launch
+// where the forward's finalize and k_adj_pixels took two (5 us each at 256^2).
+__global__ __launch_bounds__(kAdjTile) void k_adj_pixels_split(Geom g, const float4* __restrict__ partial,
+                                                               int splits, float2* __restrict__ field,
+                                                               const float* __restrict__ target, int tiles_y,
+                                                               float4* __restrict__ pa, float4* __restrict__ pb,
+                                                               float2* __restrict__ pc,
+                                                               double* __restrict__ loss_tile)
+{
+    __shared__ double red[kAdjTile];
+    const int t = blockIdx.x, l = threadIdx.x;
+    const int x = (t / tiles_y) * 16 + (l >> 4), y = (t % tiles_y) * 16 + (l & 15);
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 c2 = make_float2(1.f, 0.f);
+    double res2 = 0.0;
+    if (x < g.n0 && y < g.n1) {
+        const size_t npix = (size_t)g.n0 * g.n1, idx = (size_t)x * g.n1 + y;
+        double re, im;
+        split_sum(partial, splits, npix, idx, re, im);
+        const Frame f = pixel_frame(g, (double)x, (double)y);
+        double cs = 1.0, sn = 0.0;
+        if (g.nf) pixel_phasor(g, f, cs, sn);
+        const float2 E = split_field(g, f, cs, sn, re, im);
+        if (field != nullptr) field[idx] = E;
+        adj_pixel_terms(g, t, tiles_y, x, y, f, cs, sn, E, nullptr, target, a, b, c2, res2);
     }
+    adj_pixel_store(t, l, a, b, c2, res2, true, pa, pb, pc, loss_tile, red);
 }
 
 // HAS_W: lam_p W_cj enters the phase (nf or secondorder, not folded into A).
@@ -397,16 +446,21 @@ AdjointLayout adjoint_layout(const Geom& g, int64_t n, int num_cus, int blocks_p
 hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& L, const double* emit,
                           const double* amp, int64_t n, const float2* field, const float* dLdI, const float* target,
                           double* loss, void* scratch, float* grad_pos, float* grad_phase, float* grad_amp,
-                          hipStream_t st)
+                          hipStream_t st, const float4* fwd_partial, int fwd_splits, float2* field_out)
 {
+    if (fwd_partial != nullptr && target == nullptr) return hipErrorInvalidValue;
     const size_t np = (size_t)L.ntiles * kAdjTile;
     float4* pa = static_cast<float4*>(scratch);
     float4* pb = pa + np;
     float2* pc = reinterpret_cast<float2*>(pb + np);
     double* partial = reinterpret_cast<double*>(static_cast<char*>(scratch) + L.pix_bytes);
     double* loss_tile = reinterpret_cast<double*>(static_cast<char*>(scratch) + L.pix_bytes + L.part_bytes);
-    hipLaunchKernelGGL(k_adj_pixels, dim3((unsigned)L.ntiles), dim3(kAdjTile), 0, st, g, field, dLdI, target,
-                       L.tiles_y, pa, pb, pc, loss_tile);
+    if (fwd_partial != nullptr)
+        hipLaunchKernelGGL(k_adj_pixels_split, dim3((unsigned)L.ntiles), dim3(kAdjTile), 0, st, g, fwd_partial,
+                           fwd_splits, field_out, target, L.tiles_y, pa, pb, pc, loss_tile);
+    else
+        hipLaunchKernelGGL(k_adj_pixels, dim3((unsigned)L.ntiles), dim3(kAdjTile), 0, st, g, field, dLdI, target,
+                           L.tiles_y, pa, pb, pc, loss_tile);
     const dim3 grid((unsigned)L.egroups, (unsigned)L.splits);
     const bool a = grad_amp != nullptr;
     if (!w) {

@@ -111,6 +111,7 @@ struct idisim_handle : idisim::Planner {
     idisim_plan plan{};
     // set-up tables of the single-shot mirror-pair variants (geometry only), built on first use
     std::map<int, float*> pair_tabs;
+    void* field_tmp = nullptr;  // idisim_loss_grad_l2's field when the caller passes none
 };
 
 namespace {
@@ -323,6 +324,20 @@ int run_forward(idisim_handle* h, const FwdCall& c, int tile, int splits)
     return IDISIM_OK;
 }
 
+// The mirror-pair forward of c into float4 split partials [splits][npix] only (no finalize):
+// idisim_loss_grad_l2 sums them in its adjoint pixel stage.
+int run_forward_partials(idisim_handle* h, const FwdCall& c, int tile, int splits, float4* partial)
+{
+    bool w = false;
+    const Geom gl = split_geom(h, tile, &w);
+    const bool a = (c.amp != nullptr) || (h->g.nf && h->g.scale);
+    if (w) return fail(IDISIM_E_UNSUPPORTED, "the mirror-pair tiles need the curvature folded into A");
+    const float* tab = nullptr;
+    if (int rc = pair_table(h, c, tile, &tab)) return rc;
+    IDISIM_HIP(launch_fwd_pair(tile, a, gl, c.emit, c.amp, c.n, splits, tab, partial, nullptr, c.kind, c.st));
+    return IDISIM_OK;
+}
+
 std::vector<int> split_candidates(const idisim_handle* h, int64_t n, int tile)
 {
     const int64_t maxs = std::max<int64_t>(1, (n + kSplitGrain - 1) / kSplitGrain);
@@ -348,10 +363,11 @@ int heuristic_split(const idisim_handle* h, int64_t n, int tile)
     return c.back();
 }
 
-int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
+// The (tile, split) of a forward call: forced by the options, cached, autotuned (which runs
+// candidate forwards into c.out), or the heuristic.
+int plan_choose(idisim_handle* h, const FwdCall& c, const std::vector<int>& tiles, Choice& ch)
 {
     const TuneKey key{c.kind, c.amp != nullptr, 0, c.split_path ? 1 : 0, nbucket(c.n)};
-    Choice ch;
     auto it = h->tuned.find(key);
     const bool forced = h->force_tile >= 0 || h->force_split > 0;
     if (forced) {
@@ -437,14 +453,26 @@ int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
         const size_t per = (size_t)h->g.n0 * h->g.n1 * (c.split_path ? sizeof(float4) : sizeof(double4));
         ch.split = (int)std::max<size_t>(1, std::min<size_t>((size_t)ch.split, c.fixed_cap / per));
     }
-    int rc = run_forward(h, c, ch.tile, ch.split);
-    if (rc) return rc;
+    return IDISIM_OK;
+}
+
+// The plan a forward ran with, for idisim_get_plan.
+void plan_record(idisim_handle* h, const FwdCall& c, const Choice& ch)
+{
     if (c.split_path) h->plan.phase_bound = plan_bound(h, ch.tile, h->rho);  // the tile that ran
     h->plan.tile = ch.tile;
     h->plan.tile_rows = tile_rows(ch.tile);
     h->plan.tile_cols = tile_cols(ch.tile);
     h->plan.split = ch.split;
     h->plan.tuned_ms = ch.ms;
+}
+
+int plan_and_run(idisim_handle* h, FwdCall c, const std::vector<int>& tiles)
+{
+    Choice ch;
+    if (int rc = plan_choose(h, c, tiles, ch)) return rc;
+    if (int rc = run_forward(h, c, ch.tile, ch.split)) return rc;
+    plan_record(h, c, ch);
     return IDISIM_OK;
 }
 
@@ -743,6 +771,7 @@ int idisim_destroy(idisim_handle* h)
     if (h->h_viol) (void)hipHostFree(h->h_viol);
     for (auto& kv : h->pair_tabs)
         if (kv.second) (void)hipFree(kv.second);
+    if (h->field_tmp) (void)hipFree(h->field_tmp);
     delete h;
     return IDISIM_OK;
 }
@@ -1137,14 +1166,23 @@ int idisim_forward_batched(idisim_handle* h, const double* emitters, const doubl
     return IDISIM_OK;
 }
 
-// idisim_adjoint (target NULL) and idisim_adjoint_l2
+// The forward that idisim_loss_grad_l2 runs inside its adjoint call: its planned mirror-pair
+// tile and split; the field is summed from its partials (field may then be NULL: not stored).
+struct FusedForward {
+    FwdCall c;
+    Choice ch;
+};
+
+// idisim_adjoint (target NULL), idisim_adjoint_l2 and (fz) idisim_loss_grad_l2
 static int adjoint_call(const char* who, idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n,
                         double extent, const void* field, const float* dLdI, const float* target, double* loss,
-                        float* grad_pos, float* grad_phase, float* grad_amp, void* stream)
+                        float* grad_pos, float* grad_phase, float* grad_amp, void* stream,
+                        const FusedForward* fz = nullptr)
 {
     g_err.clear();
     std::string msg;
-    if (int rc = check_adjoint(who, h, emitters, n, field, target ? target : dLdI, grad_pos, grad_phase, extent, &msg))
+    if (int rc = check_adjoint(who, h, emitters, n, fz ? static_cast<const void*>(target) : field,
+                               target ? target : dLdI, grad_pos, grad_phase, extent, &msg))
         return fail(rc, msg);
     if (h->precision != IDISIM_PREC_FP32SPLIT)
         return fail(IDISIM_E_UNSUPPORTED, std::string(who) + ": only FP32-split handles have an adjoint");
@@ -1176,10 +1214,20 @@ static int adjoint_call(const char* who, idisim_handle* h, const double* emitter
     const Geom gl = split_geom(h, adj_tile, &w);
     const bool lam = w && !gl.nf;
     const AdjointLayout L = adjoint_layout(h->g, n, h->num_cus, adjoint_occupancy(w, lam, grad_amp != nullptr));
-    int rc = ensure_scratch(h, L.bytes);
-    if (rc) return rc;
+    float4* fpart = nullptr;
+    if (fz != nullptr) {
+        // the forward's partials after the adjoint's scratch
+        const size_t off = (L.bytes + 255) / 256 * 256;
+        const size_t fbytes = (size_t)fz->ch.split * h->g.n0 * h->g.n1 * sizeof(float4);
+        if (int rc = ensure_scratch(h, off + fbytes)) return rc;
+        fpart = reinterpret_cast<float4*>(static_cast<char*>(h->scratch) + off);
+        if (int rc = run_forward_partials(h, fz->c, fz->ch.tile, fz->ch.split, fpart)) return rc;
+    } else if (int rc = ensure_scratch(h, L.bytes)) {
+        return rc;
+    }
     IDISIM_HIP(launch_adjoint(w, lam, gl, L, emitters, amplitudes, n, static_cast<const float2*>(field), dLdI,
-                              target, loss, h->scratch, grad_pos, grad_phase, grad_amp, st));
+                              target, loss, h->scratch, grad_pos, grad_phase, grad_amp, st, fpart,
+                              fz ? fz->ch.split : 0, fz ? static_cast<float2*>(const_cast<void*>(field)) : nullptr));
     const int splits = L.splits;
     h->plan.precision = IDISIM_PREC_FP32SPLIT;
     h->plan.tile = adj_tile;
@@ -1206,6 +1254,55 @@ int idisim_adjoint_l2(idisim_handle* h, const double* emitters, const double* am
 {
     return adjoint_call("idisim_adjoint_l2", h, emitters, amplitudes, n, extent, field, nullptr, target, loss,
                         grad_pos, grad_phase, grad_amp, stream);
+}
+
+int idisim_loss_grad_l2(idisim_handle* h, const double* emitters, const double* amplitudes, int64_t n, double extent,
+                        const float* target, void* field, double* loss, float* grad_pos, float* grad_phase,
+                        float* grad_amp, void* stream)
+{
+    const char* who = "idisim_loss_grad_l2";
+    g_err.clear();
+    if (!h) return fail(IDISIM_E_INVALID, std::string(who) + ": NULL handle");
+    if (n < 1 || !emitters || !target || !grad_pos || !grad_phase)
+        return fail(IDISIM_E_INVALID, std::string(who) + ": bad arguments (n >= 1, emitters, target and the gradients)");
+    if (!std::isfinite(extent)) return fail(IDISIM_E_INVALID, std::string(who) + ": extent must be finite");
+    if (h->precision != IDISIM_PREC_FP32SPLIT)
+        return fail(IDISIM_E_UNSUPPORTED, std::string(who) + ": only FP32-split handles have an adjoint");
+    DeviceGuard guard(h->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int rc = check_extent_flag(h, who)) return rc;
+    double rho = extent;
+    if (!(rho > 0.0)) {
+        if (int rc = measure_extent(h, emitters, n, st, &rho)) return rc;
+    }
+    std::vector<int> tiles;
+    bool split_path = true;
+    if (int rc = choose_path(h, rho, &tiles, &split_path)) return rc;
+    // the field: the caller's, or the handle's buffer (the autotuner and a forward without
+    // split partials write one)
+    void* fbuf = field;
+    if (fbuf == nullptr) {
+        if (!h->field_tmp) {
+            const size_t fb = (size_t)h->g.n0 * h->g.n1 * sizeof(float2);
+            if (hipMalloc(&h->field_tmp, fb) != hipSuccess) {
+                h->field_tmp = nullptr;
+                return fail(IDISIM_E_NOMEM, std::string(who) + ": field buffer");
+            }
+        }
+        fbuf = h->field_tmp;
+    }
+    FusedForward fz{FwdCall{emitters, amplitudes, n, fbuf, IDISIM_OUT_FIELD, st, split_path, false}, Choice{}};
+    if (int rc = plan_choose(h, fz.c, tiles, fz.ch)) return rc;
+    const bool fused = split_path && tile_pair(fz.ch.tile) && fz.ch.tile != kTileBlock && fz.ch.split > 1;
+    if (!fused) {  // the forward with its own finalize, then idisim_adjoint_l2
+        if (int rc = run_forward(h, fz.c, fz.ch.tile, fz.ch.split)) return rc;
+        plan_record(h, fz.c, fz.ch);
+        return adjoint_call(who, h, emitters, amplitudes, n, rho, fbuf, nullptr, target, loss, grad_pos, grad_phase,
+                            grad_amp, stream);
+    }
+    plan_record(h, fz.c, fz.ch);
+    return adjoint_call(who, h, emitters, amplitudes, n, rho, field, nullptr, target, loss, grad_pos, grad_phase,
+                        grad_amp, stream, &fz);
 }
 
 }  // extern "C"

@@ -91,11 +91,14 @@ AdjointLayout adjoint_layout(const Geom& g, int64_t n, int num_cus, int blocks_p
 int adjoint_occupancy(bool w, bool lam, bool amp);  // resident workgroups per CU
 // w: lam_p W_cj enters the phase (not folded); lam: far-field secondorder with w.
 // target == nullptr: dL/dI is dLdI; else dL/dI = 2 (|E|^2 - target), and loss (device, may be
-// NULL) receives sum (|E|^2 - target)^2.
+// NULL) receives sum (|E|^2 - target)^2.  fwd_partial (with a target): the field is not read
+// from `field` but summed from a mirror-pair forward's float4 split partials [fwd_splits][npix],
+// as the forward's finalize would, and stored to field_out when that is not NULL.
 hipError_t launch_adjoint(bool w, bool lam, const Geom& g, const AdjointLayout& L, const double* emit,
                           const double* amp, int64_t n, const float2* field, const float* dLdI, const float* target,
                           double* loss, void* scratch, float* grad_pos, float* grad_phase, float* grad_amp,
-                          hipStream_t st);
+                          hipStream_t st, const float4* fwd_partial = nullptr, int fwd_splits = 0,
+                          float2* field_out = nullptr);
 
 hipError_t launch_extent(const double* emit, int64_t n, unsigned long long* out, hipStream_t st);
 // issue-rate probe (kind 0: v_fma_f32, 1: v_sin_f32 / v_cos_f32); out: blocks x 256 floats

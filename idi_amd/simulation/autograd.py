@@ -10,8 +10,9 @@ pos (N, 3) and phase (N,) (and optionally amp (N,)) are CUDA tensors; the
 returned intensity is float32 (N0, N1).  The backward pass calls
 idisim_adjoint, which computes dL/dpos, dL/dphase and dL/damp from dL/dI and
 the saved complex64 field (no reference counterpart).  intensity_l2_loss fuses the
-squared-intensity loss: one forward kernel and one adjoint pass (idisim_adjoint_l2, which forms
-dL/dI = 2 (I - T) itself) give the loss and the gradients; the target is a constant.
+squared-intensity loss: one call (idisim_loss_grad_l2: the forward kernel and the adjoint pass,
+which forms dL/dI = 2 (I - T) itself from the forward's partial sums) gives the loss and the
+gradients; the target is a constant.
 """
 from __future__ import annotations
 
@@ -62,9 +63,8 @@ class _IntensityL2Loss(torch.autograd.Function):
         emit = torch.cat([pos.detach().to(torch.float64), phase.detach().to(torch.float64).reshape(-1, 1)],
                          1).contiguous()
         a = None if amp is None else amp.detach().to(torch.float64).contiguous()
-        field = handle.forward(emit, a, None, _lib.OUT_FIELD, extent)
-        loss, gpos, gph, gamp = handle.adjoint_l2(emit, field, target.detach().to(torch.float32).contiguous(), a,
-                                                  extent, want_amp=amp is not None)
+        loss, gpos, gph, gamp = handle.loss_grad_l2(emit, target.detach().to(torch.float32).contiguous(), a,
+                                                    extent, want_amp=amp is not None)
         ctx.dtypes = (pos.dtype, phase.dtype, None if amp is None else amp.dtype, loss.dtype)
         ctx.save_for_backward(gpos, gph, gamp if gamp is not None else torch.empty(0, device=pos.device))
         ctx.has_amp = amp is not None

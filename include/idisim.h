@@ -220,6 +220,17 @@ IDISIM_API int idisim_adjoint_l2(idisim_handle *h, const double *emitters, const
                                  int64_t n, double extent, const void *field, const float *target, double *loss,
                                  float *grad_pos, float *grad_phase, float *grad_amp, void *stream);
 
+/* The squared-intensity loss and its gradients in one call (no reference counterpart; a fitting
+   step): idisim_forward of the emitters (IDISIM_OUT_FIELD) followed by idisim_adjoint_l2 on its
+   field, with the same results bit for bit.  When the forward's plan is a mirror-pair tile with
+   emitter splits, the adjoint's pixel pass sums the forward's split partials itself: one launch
+   where the forward's finalize and the pixel pass took two.
+     field: complex64 n0 x n1, DEVICE, or NULL: receives the forward's field
+   The other arguments are those of idisim_adjoint_l2 (target and loss as there); n >= 1. */
+IDISIM_API int idisim_loss_grad_l2(idisim_handle *h, const double *emitters, const double *amplitudes,
+                                   int64_t n, double extent, const float *target, void *field, double *loss,
+                                   float *grad_pos, float *grad_phase, float *grad_amp, void *stream);
+
 /* Largest |r_j| of n emitters (device reduction; synchronises `stream`). */
 IDISIM_API int idisim_extent(const double *emitters, int64_t n, double *extent_out, void *stream);
 

@@ -23,7 +23,7 @@ def test_exports_every_declared_symbol(lib):
     from idi_amd import _lib
 
     names = _lib.declared_functions()
-    assert len(names) == 21
+    assert len(names) == 22
     for n in names:
         assert hasattr(lib, n), n
     assert lib.idisim_version() == _lib.ABI_VERSION == 2
@@ -62,6 +62,7 @@ def test_argument_errors_without_device(lib):
     assert lib.idisim_forward_batched(None, None, None, None, 1, 0.0, None, 0, None) == -1
     assert lib.idisim_adjoint(None, None, None, 0, 0.0, None, None, None, None, None, None) == -1
     assert lib.idisim_adjoint_l2(None, None, None, 0, 0.0, None, None, None, None, None, None, None) == -1
+    assert lib.idisim_loss_grad_l2(None, None, None, 0, 0.0, None, None, None, None, None, None, None) == -1
     assert lib.idisim_set_option(None, 1, 1) == -1
     assert lib.idisim_get_plan(None, None) == -1
     assert lib.idisim_extent(None, 5, None, None) == -1

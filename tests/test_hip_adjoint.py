@@ -156,6 +156,50 @@ def test_adjoint_l2_fused_loss(lib, oracle_mod, mode):
     assert max(e) <= GRAD_RTOL, e
 
 
+@pytest.mark.parametrize("case", ["nf_4096_256px", "nf_amp", "ff_secondorder", "nf_scale", "per_pixel_tile",
+                                  "one_split"])
+def test_loss_grad_l2_equals_forward_then_adjoint(lib, oracle_mod, case):
+    """idisim_loss_grad_l2 against idisim_forward + idisim_adjoint_l2 on the same handle: the loss,
+    the gradients and the field bit for bit, both where the adjoint's pixel pass sums the forward's
+    split partials itself (mirror-pair plans with splits) and where the forward finalizes on its
+    own (a per-pixel tile, or one split)."""
+    import torch
+
+    from oracle import Geometry
+
+    rng = np.random.default_rng(95)
+    if case == "nf_4096_256px":  # the adjoint workload's shape
+        g, n, r = Geometry(256, 256, 75.0, 1e5, K_UM, nf=True), 4096, 0.02
+    else:
+        g = Geometry(64, 48, 300.0, 1e5, K_UM, nf=not case.startswith("ff"), secondorder="secondorder" in case,
+                     scale="scale" in case)
+        n, r = 600, 0.02
+    at, target = _problem(rng, n, r, g, oracle_mod)
+    amp = rng.uniform(0.5, 1, n) if case in ("nf_amp", "nf_scale") else None
+    h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+    if case == "per_pixel_tile":
+        h.set_option(lib.OPT_TILE, 2)
+    if case == "one_split":
+        h.set_option(lib.OPT_SPLIT, 1)
+    d = torch.from_numpy(at).cuda()
+    da = None if amp is None else torch.from_numpy(amp).cuda()
+    T = torch.from_numpy(target).float().cuda()
+    ext = float(np.linalg.norm(at[:, :3], axis=1).max())
+    field = h.forward(d, da, None, lib.OUT_FIELD, ext)
+    ref = h.adjoint_l2(d, field, T, da, ext, want_amp=amp is not None)
+    plan_adj = h.plan()
+    f2 = torch.empty_like(field)
+    for out in (f2, None):  # with and without the field output
+        got = h.loss_grad_l2(d, T, da, ext, field=out, want_amp=amp is not None)
+        torch.cuda.synchronize()
+        assert got[0].item() == ref[0].item(), (case, got[0].item(), ref[0].item())
+        for a_, b_ in zip(got[1:], ref[1:]):
+            if b_ is not None:
+                assert torch.equal(a_, b_), case
+    assert torch.equal(f2, field), case
+    assert h.plan()["split"] == plan_adj["split"]  # the adjoint's plan, as after adjoint_l2
+
+
 def test_autograd_l2_loss(lib, oracle_mod):
     import torch
 
