@@ -33,7 +33,7 @@ def test_bench_reads_the_mix_of_the_plan():
     import bench
 
     m = bench.load_mix("single", 13)
-    assert m["valu"] == pytest.approx(5.747) and m["trans"] == pytest.approx(0.519)
+    assert m["valu"] == pytest.approx(5.759) and m["trans"] == pytest.approx(0.519)
     assert bench.load_mix("single", 4) is None
     # frac counts issue slots (a transcendental = 4) at two flops per slot: at most 1 at the peak rate
     slots = m["valu"] - m["trans"] + 4 * m["trans"]
