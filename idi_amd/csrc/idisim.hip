@@ -1291,7 +1291,8 @@ int idisim_loss_grad_l2(idisim_handle* h, const double* emitters, const double* 
         }
         fbuf = h->field_tmp;
     }
-    FusedForward fz{FwdCall{emitters, amplitudes, n, fbuf, IDISIM_OUT_FIELD, st, split_path, false}, Choice{}};
+    // an FP64 fallback forward writes the complex64 field through its finalize (out_f32)
+    FusedForward fz{FwdCall{emitters, amplitudes, n, fbuf, IDISIM_OUT_FIELD, st, split_path, !split_path}, Choice{}};
     if (int rc = plan_choose(h, fz.c, tiles, fz.ch)) return rc;
     const bool fused = split_path && tile_pair(fz.ch.tile) && fz.ch.tile != kTileBlock && fz.ch.split > 1;
     if (!fused) {  // the forward with its own finalize, then idisim_adjoint_l2

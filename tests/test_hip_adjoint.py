@@ -200,6 +200,32 @@ def test_loss_grad_l2_equals_forward_then_adjoint(lib, oracle_mod, case):
     assert h.plan()["split"] == plan_adj["split"]  # the adjoint's plan, as after adjoint_l2
 
 
+def test_loss_grad_l2_fp64_fallback_geometry(lib, oracle_mod):
+    """Pixels of 2 mm at 0.1 m: the forward falls back to the FP64 kernel (complex64 field through
+    its finalize) and the adjoint, which has no FP64 form, refuses the geometry; the one call does
+    what the two calls do."""
+    import torch
+
+    from oracle import Geometry
+
+    rng = np.random.default_rng(96)
+    g = Geometry(24, 20, 2000.0, 1e5, K_UM, nf=True)
+    at, target = _problem(rng, 300, 0.03, g, oracle_mod)
+    h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+    d = torch.from_numpy(at).cuda()
+    T = torch.from_numpy(target).float().cuda()
+    field = h.forward(d)
+    assert h.plan()["fallback"] == 1 and field.dtype == torch.complex64
+    with pytest.raises(lib.IdisimError) as e2:
+        h.adjoint_l2(d, field, T)
+    f2 = torch.zeros_like(field)
+    with pytest.raises(lib.IdisimError) as e1:
+        h.loss_grad_l2(d, T, field=f2)
+    assert e1.value.code == e2.value.code == lib.E_PRECISION
+    torch.cuda.synchronize()
+    assert torch.equal(f2, field)  # the forward ran, into the caller's complex64 buffer
+
+
 def test_autograd_l2_loss(lib, oracle_mod):
     import torch
 
