@@ -436,6 +436,18 @@ int plan_choose(idisim_handle* h, const FwdCall& c, const std::vector<int>& tile
             ch.split = std::min(ch.split, full.back());
         }
         h->tuned[key] = ch;
+        // the set-up tables of the candidates that lost (another call may build one again)
+        for (auto it2 = h->pair_tabs.begin(); it2 != h->pair_tabs.end();) {
+            bool keep = it2->first == ch.tile;
+            for (const auto& kv : h->tuned) keep = keep || kv.second.tile == it2->first;
+            if (keep) {
+                ++it2;
+            } else {
+                IDISIM_HIP(hipStreamSynchronize(c.st));
+                (void)hipFree(it2->second);
+                it2 = h->pair_tabs.erase(it2);
+            }
+        }
     } else {
         // heuristic: a mirror-pair quad per lane if allowed, enough splits for 4 WG/CU
         ch.tile = tiles.front();
