@@ -200,6 +200,62 @@ def test_loss_grad_l2_equals_forward_then_adjoint(lib, oracle_mod, case):
     assert h.plan()["split"] == plan_adj["split"]  # the adjoint's plan, as after adjoint_l2
 
 
+@pytest.mark.parametrize("mode", ["nf", "ff_secondorder", "nf_scale"])
+def test_adjoint_ragged_detector(lib, oracle_mod, mode):
+    """Detectors that are no multiple of 16 or 32 pixels (odd sizes, so the centre falls between
+    pixels): the last row and column of 16 x 16 adjoint tiles and of 32 x 32 forward tiles are
+    partly outside the detector.  The gradients against the FP64 analytic gradient, the loss against
+    the oracle's, and the one-call loss and gradient against forward + idisim_adjoint_l2 bit for bit.
+    On the 300 um detector the planner's forward is a per-pixel tile (its own finalize); the 75 um one
+    pins the centred mirror-pair tile 11 with 4 splits (within the 5e-6 rev bound there), so the
+    adjoint's pixel pass sums the forward's split partials over the ragged tiles itself."""
+    import torch
+
+    from oracle import Geometry
+
+    rng = np.random.default_rng(97)
+    for n0, n1, pix, n in ((50, 37, 300.0, 500), (100, 77, 75.0, 1024)):
+        g = Geometry(n0, n1, pix, 1e5, K_UM, nf=mode.startswith("nf"), secondorder="secondorder" in mode,
+                     scale="scale" in mode)
+        at, target = _problem(rng, n, 0.02, g, oracle_mod)
+        amp = rng.uniform(0.5, 1, n) if "scale" in mode else None
+        fref = oracle_mod.field_c(g, at, amp)
+        t32 = target.astype(np.float32).astype(np.float64)
+        rpos, rph, ramp = oracle_mod.adjoint_c(g, at, 2 * (np.abs(fref) ** 2 - t32), field=fref, amp=amp)
+        lref = np.sum((np.abs(fref) ** 2 - t32) ** 2)
+
+        h = make_handle(lib, g, lib.PREC_FP32SPLIT)
+        pinned = pix == 75.0
+        if pinned:
+            h.set_option(lib.OPT_TILE, 11)
+            h.set_option(lib.OPT_SPLIT, 4)
+        d = torch.from_numpy(at).cuda()
+        da = None if amp is None else torch.from_numpy(amp).cuda()
+        T = torch.from_numpy(target).float().cuda()
+        ext = float(np.linalg.norm(at[:, :3], axis=1).max())
+        field = h.forward(d, da, None, lib.OUT_FIELD, ext)
+        fplan = h.plan()  # the plan that the forward inside loss_grad_l2 runs too
+        print(mode, n0, n1, "forward plan", fplan)
+        if pinned:
+            assert (fplan["tile"], fplan["split"]) == (11, 4) and fplan["phase_bound"] <= 5e-6, fplan
+        ref = h.adjoint_l2(d, field, T, da, ext, want_amp=amp is not None)
+        f2 = torch.empty((n0, n1), dtype=torch.complex64, device="cuda")
+        got = h.loss_grad_l2(d, T, da, ext, field=f2, want_amp=amp is not None)
+        torch.cuda.synchronize()
+        assert torch.equal(f2, field), (mode, n0, n1)
+        assert got[0].item() == ref[0].item(), (mode, n0, n1)
+        for a_, b_ in zip(got[1:], ref[1:]):
+            if b_ is not None:
+                assert torch.equal(a_, b_), (mode, n0, n1)
+        assert got[0].item() == pytest.approx(lref, rel=1e-4), (mode, n0, n1)
+        e = [rel_l2(got[1].cpu().numpy().astype(np.float64), rpos),
+             rel_l2(got[2].cpu().numpy().astype(np.float64), rph)]
+        if amp is not None:
+            e.append(rel_l2(got[3].cpu().numpy().astype(np.float64), ramp))
+        print(mode, n0, n1, "relL2 vs FP64 analytic", e)
+        assert max(e) <= GRAD_RTOL, (mode, n0, n1, e)
+
+
 def test_loss_grad_l2_fp64_fallback_geometry(lib, oracle_mod):
     """Pixels of 2 mm at 0.1 m: the forward falls back to the FP64 kernel (complex64 field through
     its finalize) and the adjoint, which has no FP64 form, refuses the geometry; the one call does
