@@ -5,7 +5,8 @@ The no-alloc entry point keeps its emitter-split partial sums in the library's s
 device scratch.  Checked here:
   * two calls in a row (100000 simple-cubic emitters on the 512^2 detector at 0.2 m, then
     3000 emitters in a 10 nm ball) give the same bits as idisim_forward with the same plan,
-    for an FP32-split and an FP64 handle, field and intensity outputs;
+    for an FP32-split and an FP64 handle, field and intensity outputs; and the same on a
+    100 x 77 detector (partly filled edge tiles) for a mirror-pair and a per-pixel tile;
   * without a tuned plan (heuristic plan, measured extent) the result is within 1e-4 of the
     FP64 oracle rows;
   * the calls leave the device's free memory unchanged;
@@ -70,6 +71,34 @@ def test_noalloc_equals_forward_two_calls(lib, precision):
             pb = hb.plan()
             assert (pb["tile"], pb["split"], pb["fallback"]) == (pa["tile"], pa["split"], pa["fallback"]), (pa, pb)
             assert torch.equal(out, ref), (precision, kind, len(at))
+
+
+@pytest.mark.parametrize("tile", [11, 0])
+def test_noalloc_ragged_detector(lib, tile):
+    """A detector of 100 x 77 pixels (no multiple of 16 or 32): the partial sums of the partly
+    filled edge tiles in the static buffer, for a mirror-pair tile (11) and a per-pixel tile (0) at
+    4 splits, give idisim_forward's bits on two calls in a row (1024, then 300 emitters)."""
+    import torch
+
+    rng = np.random.default_rng(7701)
+    ha = lib.Handle(100, 77, 75.0, 1e5, K_UM, nf=True, precision=lib.PREC_FP32SPLIT)
+    hb = lib.Handle(100, 77, 75.0, 1e5, K_UM, nf=True, precision=lib.PREC_FP32SPLIT)
+    for h in (ha, hb):
+        h.set_option(lib.OPT_TILE, tile)
+        h.set_option(lib.OPT_SPLIT, 4)
+    for n in (1024, 300):
+        at = atoms_ball(rng, n, 0.02)
+        d = torch.from_numpy(at).cuda()
+        ext = float(np.linalg.norm(at[:, :3], axis=1).max())
+        for kind in (lib.OUT_FIELD, lib.OUT_INTENSITY):
+            print(f"noalloc ragged tile={tile} n={n} kind={kind}", flush=True)
+            ref = ha.forward(d, None, None, kind, ext)
+            out = torch.full_like(ref, float("nan"))
+            hb.forward_noalloc(d, out, None, kind, ext)
+            torch.cuda.synchronize()
+            p = hb.plan()
+            assert (p["tile"], p["split"], p["fallback"]) == (tile, 4, 0) and p["phase_bound"] <= 5e-6, p
+            assert torch.equal(out, ref), (tile, n, kind)
 
 
 def test_noalloc_heuristic_plan_matches_oracle(lib, oracle_mod):
