@@ -16,7 +16,10 @@
 // FP64 kernel (reference counterpart: wfkerneld, sim.cu:146-149, and the numba
 // kernel cpu.py:68-107): the literal formula in FP64 with Kahan per term, same
 // tiling and LDS staging; it is the "plain FP64" comparator and the fallback.
+#include <type_traits>
+
 #include "device_util.hpp"
+#include "sincos64.hpp"
 
 namespace idisim {
 
@@ -1113,27 +1116,40 @@ __device__ __forceinline__ void fp64_tile(const Geom& g, const double* __restric
             s1[tid] = (amp != nullptr) ? amp[e] : 1.0;
         }
         __syncthreads();
-        for (int e = 0; e < cnt; ++e) {
-            const double4 at = s0[e];
-            const double a = s1[e];
+        // one emitter loop per mode (near or far field, with or without the 1/r amplitude): the
+        // mode is uniform, and a loop that selects inside was if-converted to compute both phases
+        // (and the square root) for every pair
+        auto emitters = [&](auto nf_tag, auto scale_tag) {
+            constexpr bool NF = decltype(nf_tag)::value, SCALE = decltype(scale_tag)::value;
+            for (int e = 0; e < cnt; ++e) {
+                const double4 at = s0[e];
+                const double a = s1[e];
 #pragma unroll
-            for (int p = 0; p < P; ++p) {
-                double phase, mag;
-                if (g.nf) {
-                    const double ddx = dx[p] - at.x, ddy = dy[p] - at.y, ddz = dz[p] - at.z;
-                    const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-                    phase = (dist - dz[p] + at.z) * g.k + at.w;
-                    mag = g.scale ? a / dist : a;
-                } else {
-                    phase = -(at.x * qx[p] + at.y * qy[p] + at.z * qz[p]) + at.w;
-                    if (g.secondorder) phase += c2[p] * (at.x * at.x + at.y * at.y);
-                    mag = g.scale ? a * rd[p] : a;
+                for (int p = 0; p < P; ++p) {
+                    double phase, mag;
+                    if constexpr (NF) {
+                        const double ddx = dx[p] - at.x, ddy = dy[p] - at.y, ddz = dz[p] - at.z;
+                        const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                        phase = (dist - dz[p] + at.z) * g.k + at.w;
+                        mag = SCALE ? a / dist : a;
+                    } else {
+                        phase = -(at.x * qx[p] + at.y * qy[p] + at.z * qz[p]) + at.w;
+                        if (g.secondorder) phase += c2[p] * (at.x * at.x + at.y * at.y);
+                        mag = SCALE ? a * rd[p] : a;
+                    }
+                    double sn, cs;
+                    sincos_fp64(phase, &sn, &cs);
+                    kahan_add_d(acc_re[p], cmp_re[p], mag * cs);
+                    kahan_add_d(acc_im[p], cmp_im[p], mag * sn);
                 }
-                double sn, cs;
-                sincos(phase, &sn, &cs);
-                kahan_add_d(acc_re[p], cmp_re[p], mag * cs);
-                kahan_add_d(acc_im[p], cmp_im[p], mag * sn);
             }
+        };
+        if (g.nf) {
+            if (g.scale) emitters(std::true_type{}, std::true_type{});
+            else emitters(std::true_type{}, std::false_type{});
+        } else {
+            if (g.scale) emitters(std::false_type{}, std::true_type{});
+            else emitters(std::false_type{}, std::false_type{});
         }
     }
 }

@@ -3,12 +3,15 @@
 // compiled with a host compiler under AddressSanitizer and UndefinedBehaviorSanitizer.
 //
 // `planner_check checks` runs the argument checks on good and bad arguments and prints
-// {"name": code, ...}.  Otherwise, input, one case per line (stdin):
+// {"name": code, ...}.  `planner_check sincos` reads one double per line (stdin) and prints
+// "%a %a", the sine and cosine of the FP64 kernels' sincos_fp64 (idi_amd/csrc/sincos64.hpp).
+// Otherwise, input, one case per line (stdin):
 //   n0 n1 pixelsize detz k off0 off1 nf scale secondorder rot m0 .. m8 tol_rev summode nrho rho_1 .. rho_nrho
 // Output, one JSON object per case (stdout): the planner tables, and for every extent the
 // bound of every tile variant (null where it is infinite) and the eligible variants.
 #include "../../idi_amd/csrc/checks.hpp"
 #include "../../idi_amd/csrc/planner.hpp"
+#include "../../idi_amd/csrc/sincos64.hpp"
 
 #include <cstdio>
 #include <cstring>
@@ -103,6 +106,15 @@ static int checks()
 int main(int argc, char** argv)
 {
     if (argc > 1 && std::strcmp(argv[1], "checks") == 0) return checks();
+    if (argc > 1 && std::strcmp(argv[1], "sincos") == 0) {
+        double x;
+        while (std::scanf("%lf", &x) == 1) {
+            double sn, cs;
+            sincos_fp64(x, &sn, &cs);
+            std::printf("%a %a\n", sn, cs);
+        }
+        return 0;
+    }
     for (;;) {
         long n0, n1;
         double pix, detz, k, off0, off1, tol;

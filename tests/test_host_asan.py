@@ -173,3 +173,37 @@ def test_planner_bound_of_the_benchmark_image(driver):
     g = Geometry(512, 512, 75.0, 2e5, K_UM, nf=True)
     (r,) = _plan(driver, [g], [rho])
     assert r["extents"][0]["bound"][13] == pytest.approx(1.6306114528647848e-06, rel=1e-12)
+
+
+def test_fp64_sincos_against_mpmath(driver):
+    """sincos_fp64 of the FP64 kernels (idi_amd/csrc/sincos64.hpp), compiled for the host, against
+    40-digit mpmath: random phases of every magnitude up to 1e8 rad, arguments next to multiples
+    of pi/2 (where the reduction cancels), zero and both sides of the library fallback at
+    2^26 C1.  Each value within 2^-52 of the true sine and cosine (measured: 1.3 x 2^-53; libm 0.5);
+    a dropped C3 (1.5e6 x 2^-52), a dropped r^15 term (92 x) or a wrong quadrant sign exceeds it."""
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    rng = np.random.default_rng(5)
+    lim = float.fromhex("0x1.921fb54p+26")
+    xs = np.concatenate([
+        [0.0, -0.0, 1e-300, 1e-10, 0.5, np.pi / 4, np.pi / 2, np.pi, 3 * np.pi / 2, -np.pi / 2],
+        rng.uniform(-10, 10, 1000),
+        10 ** rng.uniform(0, 8, 1000) * rng.choice([-1, 1], 1000),
+        np.arange(-500, 500) * (np.pi / 2),
+        np.arange(1, 500) * (np.pi / 2) * 1e4,
+        [np.nextafter(lim, 0), lim, 1.2e8, -1.2e8],
+    ])
+    out = subprocess.run([driver, "sincos"], input="\n".join(repr(float(x)) for x in xs) + "\n",
+                         capture_output=True, text=True, env=_env())
+    assert out.returncode == 0, out.stderr[-3000:]
+    lines = out.stdout.split()
+    assert len(lines) == 2 * len(xs)
+    worst = 0.0
+    for i, x in enumerate(xs):
+        s, c = float.fromhex(lines[2 * i]), float.fromhex(lines[2 * i + 1])
+        X = mp.mpf(float(x))
+        err = max(abs(mp.mpf(s) - mp.sin(X)), abs(mp.mpf(c) - mp.cos(X)))
+        worst = max(worst, float(err))
+        assert err <= 2.0**-52, (x, s, c)
+    print("sincos_fp64 max abs error / 2^-53:", worst / 2.0**-53)
