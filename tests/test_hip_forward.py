@@ -421,3 +421,30 @@ def test_issue_rates(lib):
     assert 1e12 < trans < fma < 1e15
     with pytest.raises(lib.IdisimError):
         lib.issue_rates(999)
+
+
+def test_fp64_phases_on_both_sides_of_the_sincos_reduction_limit(lib, oracle_mod):
+    """The FP64 kernel's sincos reduces by pi/2 below |x| = 2^26 C1 (1.05e8 rad) and calls the library
+    above.  A 32 x 32 detector of 100 um pixels offset by 16 mm at 0.1 m spans near-field phases
+    k (|r_p| - z_p) from about 8.5e7 to 1.3e8 rad, so pixels of both sides are summed; the image
+    against the FP64 oracle within the two evaluations' rounding of k |r_p|, and the batched FP64
+    kernel's image bit for bit."""
+    import torch
+
+    from oracle import Geometry
+
+    rng = np.random.default_rng(48)
+    g = Geometry(32, 32, 100.0, 1e5, K_UM, detoffset=(16000.0, 16000.0), nf=True)
+    xs = (np.arange(32) - 16) * 100.0 + 16000.0
+    rho2 = xs[:, None] ** 2 + xs[None, :] ** 2
+    phase = K_UM * (np.sqrt(rho2 + 1e10) - 1e5)
+    lim = float.fromhex("0x1.921fb54p+26")
+    assert phase.min() < 0.9 * lim and phase.max() > 1.1 * lim, (phase.min(), phase.max())
+    at = atoms_ball(rng, 400, 0.03)
+    ref = oracle_mod.field_c(g, at)
+    f, h = run(lib, g, at, precision=lib.PREC_FP64, split=1)  # no split: the batched kernel's sums
+    e = rel_l2(f, ref)
+    print("fp64 across the sincos limit: relL2", e, "bound", fp64_tol(g))
+    assert e <= fp64_tol(g)
+    out = h.forward_batched(torch.from_numpy(at).cuda(), [0, len(at)]).cpu().numpy()
+    np.testing.assert_array_equal(out[0], f)
