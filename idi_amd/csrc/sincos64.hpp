@@ -19,6 +19,20 @@
 
 namespace idisim {
 
+// a * b + c with c a constant: v_fma_f64 with the constant in an SGPR pair.  The compiler's
+// choice for a Horner step is v_fmac_f64, whose destination is its addend, so every step copied
+// the constant into a fresh register first (one v_mov_b64 per step).
+__host__ __device__ inline double fma_const(double a, double b, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+#else
+    return fma(a, b, c);
+#endif
+}
+
 __host__ __device__ inline void sincos_fp64(double x, double* s, double* c)
 {
     constexpr double kC1 = 0x1.921fb54p+0;
@@ -37,23 +51,23 @@ __host__ __device__ inline void sincos_fp64(double x, double* s, double* c)
     const double r2 = r * r;
     // sin r = r + r^3 (-1/3! + r^2 (1/5! + r^2 (... + r^2 / 17!)))
     double ps = 1.0 / 355687428096000.0;
-    ps = fma(ps, r2, -1.0 / 1307674368000.0);
-    ps = fma(ps, r2, 1.0 / 6227020800.0);
-    ps = fma(ps, r2, -1.0 / 39916800.0);
-    ps = fma(ps, r2, 1.0 / 362880.0);
-    ps = fma(ps, r2, -1.0 / 5040.0);
-    ps = fma(ps, r2, 1.0 / 120.0);
-    ps = fma(ps, r2, -1.0 / 6.0);
+    ps = fma_const(ps, r2, -1.0 / 1307674368000.0);
+    ps = fma_const(ps, r2, 1.0 / 6227020800.0);
+    ps = fma_const(ps, r2, -1.0 / 39916800.0);
+    ps = fma_const(ps, r2, 1.0 / 362880.0);
+    ps = fma_const(ps, r2, -1.0 / 5040.0);
+    ps = fma_const(ps, r2, 1.0 / 120.0);
+    ps = fma_const(ps, r2, -1.0 / 6.0);
     const double sr = fma(r * r2, ps, r);
     // cos r = 1 + r^2 (-1/2 + r^2 (1/4! + r^2 (... + r^2 / 16!)))
     double pc = 1.0 / 20922789888000.0;
-    pc = fma(pc, r2, -1.0 / 87178291200.0);
-    pc = fma(pc, r2, 1.0 / 479001600.0);
-    pc = fma(pc, r2, -1.0 / 3628800.0);
-    pc = fma(pc, r2, 1.0 / 40320.0);
-    pc = fma(pc, r2, -1.0 / 720.0);
-    pc = fma(pc, r2, 1.0 / 24.0);
-    pc = fma(pc, r2, -0.5);
+    pc = fma_const(pc, r2, -1.0 / 87178291200.0);
+    pc = fma_const(pc, r2, 1.0 / 479001600.0);
+    pc = fma_const(pc, r2, -1.0 / 3628800.0);
+    pc = fma_const(pc, r2, 1.0 / 40320.0);
+    pc = fma_const(pc, r2, -1.0 / 720.0);
+    pc = fma_const(pc, r2, 1.0 / 24.0);
+    pc = fma_const(pc, r2, -0.5);
     const double cr = fma(r2, pc, 1.0);
     // x = n pi/2 + r: quadrant q = n mod 4 (two's complement, also for n < 0)
     const int q = (int)n & 3;
